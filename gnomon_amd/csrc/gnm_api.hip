@@ -1,0 +1,790 @@
+// Host side of the C ABI declared in include/gnomon_hip.h.
+//
+// gnm_create turns the reference's CSR weight matrix (PreparationResult,
+// score/types.rs:166-365) into a device plan:
+//   * effect-allele rows: a variant whose CSR entries are all unflipped
+//     (corr == 0) becomes one A2-dosage row with weights W_eff; one whose
+//     entries are all flipped becomes one A1-dosage row with weights corr/2
+//     (the original effect weight, score/prepare.rs:197-208); a variant with
+//     both kinds gets both rows.  In effect-allele space a missing call
+//     contributes 0, which equals baseline + W_eff*d - corr-on-missing.
+//   * membership classes: scores whose CSR columns hold the same variant set
+//     share one missing-count accumulator.
+//   * FP32 weights (plain path) or split-BF16 integer limbs with per
+//     (scale block, column) power-of-two scales, pre-arranged in the lane
+//     order of v_mfma_f32_16x16x32_bf16 (MFMA path).
+// Rows then stream through gnm_score_rows (pinned staging, asynchronous
+// copies) or arrive device-resident; partial sums are reduced in FP64 into
+// persistent device accumulators.
+#include "../../include/gnomon_hip.h"
+#include "gnm_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+using namespace gnm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+gnm_status fail(gnm_status st, const std::string &msg) {
+    g_last_error = msg;
+    return st;
+}
+
+#define GNM_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            return fail(e_ == hipErrorOutOfMemory ? GNM_ERR_OUT_OF_MEMORY : GNM_ERR_HIP, \
+                        std::string(#call) + ": " + hipGetErrorString(e_));            \
+        }                                                                              \
+    } while (0)
+
+uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+bool env_hides_devices() {
+    for (const char *name : {"HIP_VISIBLE_DEVICES", "ROCR_VISIBLE_DEVICES", "CUDA_VISIBLE_DEVICES"}) {
+        const char *v = std::getenv(name);
+        if (!v) continue;
+        std::string s(v);
+        while (!s.empty() && (s.back() == ' ' || s.back() == '\t')) s.pop_back();
+        while (!s.empty() && (s.front() == ' ' || s.front() == '\t')) s.erase(s.begin());
+        std::string low = s;
+        for (auto &ch : low) ch = (char)std::tolower((unsigned char)ch);
+        if (s.empty() || s == "-1" || low == "none") return true;
+    }
+    return false;
+}
+
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct gnm_ctx {
+    // problem
+    uint64_t N = 0, Nfam = 0, bpv = 0, K = 0, V = 0;
+    int precision = GNM_PRECISION_BF16X3;
+    int device = 0;
+    bool identity = true;
+    // host copies needed after create
+    std::vector<uint64_t> var_prog_off;   // [V+1]
+    std::vector<uint64_t> row_offsets;    // [V+1] CSR
+    std::vector<uint32_t> cols;           // [nnz]
+    std::vector<float> corr;              // [nnz]
+    std::vector<uint32_t> score_class;    // [K]
+    uint64_t R = 0;                       // program rows
+    uint32_t nclass = 0;
+    uint32_t limbs = 1;
+    bool fp32_count_in_kernel = true;
+    // device
+    DevicePlan plan{};
+    std::vector<DevBuf> allocations;
+    uint32_t *d_score_class = nullptr;
+    uint32_t *d_out_to_fam = nullptr;
+    double *d_scores = nullptr;
+    uint32_t *d_counts = nullptr;
+    double *d_part = nullptr;
+    uint32_t *d_cpart = nullptr;
+    uint32_t splits = 8;
+    uint8_t *d_compact = nullptr;
+    uint64_t compact_rows = 0;     // capacity in rows
+    uint64_t cbpv = 0;             // compact bytes per variant
+    uint8_t *d_stage[2] = {nullptr, nullptr};
+    uint8_t *h_stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_free[2] = {nullptr, nullptr};
+    uint64_t stage_rows = 0;       // capacity of each staging slot in rows
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_main0 = nullptr, ev_main1 = nullptr, ev_all0 = nullptr, ev_all1 = nullptr;
+    bool timing_valid = false;
+    uint64_t completed = 0;
+
+    template <class T>
+    gnm_status alloc(T **out, size_t count) {
+        void *p = nullptr;
+        const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess)
+            return fail(e == hipErrorOutOfMemory ? GNM_ERR_OUT_OF_MEMORY : GNM_ERR_HIP,
+                        "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+        allocations.push_back({p, bytes});
+        *out = (T *)p;
+        return GNM_OK;
+    }
+    template <class T>
+    gnm_status upload(T **out, const std::vector<T> &host) {
+        gnm_status st = alloc(out, host.size());
+        if (st != GNM_OK) return st;
+        if (!host.empty()) GNM_HIP(hipMemcpy(*out, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
+        return GNM_OK;
+    }
+};
+
+namespace {
+
+gnm_status validate(const gnm_problem *p) {
+    if (!p) return fail(GNM_ERR_INVALID_ARG, "problem is NULL");
+    if (p->num_people == 0 || p->num_scores == 0)
+        return fail(GNM_ERR_INVALID_ARG, "num_people and num_scores must be positive");
+    if (p->total_people_in_fam < p->num_people)
+        return fail(GNM_ERR_INVALID_ARG, "total_people_in_fam < num_people");
+    if (p->bytes_per_variant != (p->total_people_in_fam + 3) / 4)
+        return fail(GNM_ERR_INVALID_ARG, "bytes_per_variant must equal ceil(total_people_in_fam/4)");
+    if (p->num_variants > 0xFFFFFFF0ull) return fail(GNM_ERR_INVALID_ARG, "too many variants");
+    if (!p->sparse_row_offsets) return fail(GNM_ERR_INVALID_ARG, "sparse_row_offsets is NULL");
+    if (p->sparse_row_offsets[0] != 0 || p->sparse_row_offsets[p->num_variants] != p->nnz)
+        return fail(GNM_ERR_INVALID_ARG, "sparse_row_offsets must start at 0 and end at nnz");
+    if (p->nnz && (!p->sparse_weights || !p->sparse_missing_corrections || !p->sparse_score_columns))
+        return fail(GNM_ERR_INVALID_ARG, "CSR arrays are NULL");
+    for (uint64_t v = 0; v < p->num_variants; ++v)
+        if (p->sparse_row_offsets[v + 1] < p->sparse_row_offsets[v])
+            return fail(GNM_ERR_INVALID_ARG, "sparse_row_offsets must be non-decreasing");
+    for (uint64_t e = 0; e < p->nnz; ++e)
+        if (p->sparse_score_columns[e] >= p->num_scores)
+            return fail(GNM_ERR_INVALID_ARG, "CSR score column out of range");
+    if (!p->output_idx_to_fam_idx && p->num_people != p->total_people_in_fam)
+        return fail(GNM_ERR_INVALID_ARG, "output_idx_to_fam_idx is required when num_people < total_people_in_fam");
+    if (p->output_idx_to_fam_idx)
+        for (uint64_t i = 0; i < p->num_people; ++i)
+            if (p->output_idx_to_fam_idx[i] >= p->total_people_in_fam)
+                return fail(GNM_ERR_INVALID_ARG, "output_idx_to_fam_idx entry out of range");
+    if (p->precision < GNM_PRECISION_FP32 || p->precision > GNM_PRECISION_AUTO)
+        return fail(GNM_ERR_INVALID_ARG, "unknown precision");
+    for (int i = 0; i < 4; ++i)
+        if (p->reserved[i]) return fail(GNM_ERR_INVALID_ARG, "reserved fields must be zero");
+    return GNM_OK;
+}
+
+// Signed base-256 digits of an integer with |Q| <= 2^(8L) - 2^(8(L-1)):
+// low digits in [-128, 127], top digit in [-255, 255]; all exact in BF16.
+inline void limb_digits(int64_t Q, uint32_t L, int32_t *out) {
+    for (uint32_t j = L - 1; j > 0; --j) {
+        int64_t d = ((Q + 128) & 255) - 128;
+        out[j] = (int32_t)d;
+        Q = (Q - d) / 256;
+    }
+    out[0] = (int32_t)Q;
+}
+
+inline uint16_t bf16_of_int(int32_t q) {
+    // |q| <= 256: exactly representable; float -> bf16 by truncation is exact.
+    float f = (float)q;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return (uint16_t)(u >> 16);
+}
+
+template <class F>
+void parallel_for(uint64_t n, F &&fn) {
+    unsigned T = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    if (n < 64 || T == 1) {
+        fn(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    uint64_t chunk = (n + T - 1) / T;
+    for (unsigned t = 0; t < T; ++t) {
+        uint64_t a = t * chunk, b = std::min(n, a + chunk);
+        if (a >= b) break;
+        th.emplace_back([&fn, a, b] { fn(a, b); });
+    }
+    for (auto &x : th) x.join();
+}
+
+gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
+    c->N = p->num_people;
+    c->Nfam = p->total_people_in_fam;
+    c->bpv = p->bytes_per_variant;
+    c->K = p->num_scores;
+    c->V = p->num_variants;
+    c->precision = p->precision == GNM_PRECISION_AUTO ? GNM_PRECISION_BF16X3 : p->precision;
+    c->device = p->device < 0 ? 0 : p->device;
+    c->limbs = c->precision == GNM_PRECISION_BF16X2 ? 2 : (c->precision == GNM_PRECISION_BF16X3 ? 3 : 1);
+    const uint64_t V = c->V, K = c->K, nnz = p->nnz;
+
+    c->identity = true;
+    if (p->output_idx_to_fam_idx) {
+        if (c->N != c->Nfam) c->identity = false;
+        for (uint64_t i = 0; i < c->N && c->identity; ++i)
+            if (p->output_idx_to_fam_idx[i] != i) c->identity = false;
+    }
+
+    c->row_offsets.assign(p->sparse_row_offsets, p->sparse_row_offsets + V + 1);
+    c->cols.assign(p->sparse_score_columns, p->sparse_score_columns + nnz);
+    c->corr.assign(p->sparse_missing_corrections, p->sparse_missing_corrections + nnz);
+
+    // ---- membership classes (identical CSR column supports share a count)
+    std::vector<uint64_t> h1(K, 0), h2(K, 0), cnt(K, 0);
+    for (uint64_t v = 0; v < V; ++v) {
+        const uint64_t a = mix64(v ^ 0x5851F42D4C957F2Dull), b = mix64(v + 0x14057B7EF767814Full);
+        for (uint64_t e = c->row_offsets[v]; e < c->row_offsets[v + 1]; ++e) {
+            const uint32_t k = c->cols[e];
+            h1[k] += a;
+            h2[k] ^= b + (h1[k] << 7);  // order-dependent mix is fine: variants arrive ascending
+            cnt[k] += 1;
+        }
+    }
+    std::map<std::tuple<uint64_t, uint64_t, uint64_t>, uint32_t> cls_of;
+    c->score_class.resize(K);
+    for (uint64_t k = 0; k < K; ++k) {
+        auto key = std::make_tuple(cnt[k], h1[k], h2[k]);
+        auto it = cls_of.find(key);
+        if (it == cls_of.end()) it = cls_of.emplace(key, (uint32_t)cls_of.size()).first;
+        c->score_class[k] = it->second;
+    }
+    c->nclass = (uint32_t)cls_of.size();
+
+    // ---- effect-allele program rows
+    c->var_prog_off.assign(V + 1, 0);
+    std::vector<uint8_t> var_rows(V, 0);  // bit0: A2 row, bit1: A1 row
+    for (uint64_t v = 0; v < V; ++v) {
+        bool need_d = false, need_e = false;
+        for (uint64_t e = c->row_offsets[v]; e < c->row_offsets[v + 1]; ++e) {
+            const double we = p->sparse_weights[e], co = p->sparse_missing_corrections[e];
+            if (co != 0.0) need_e = true;
+            if (we + 0.5 * co != 0.0) need_d = true;
+        }
+        uint8_t m = need_e ? (need_d ? 3 : 2) : 1;
+        var_rows[v] = m;
+        c->var_prog_off[v + 1] = c->var_prog_off[v] + (m == 3 ? 2 : 1);
+    }
+    const uint64_t R = c->var_prog_off[V];
+    c->R = R;
+    const uint64_t R32 = (R + kKStep - 1) / kKStep * kKStep;
+    const uint64_t Rpad = std::max<uint64_t>(R32, kKStep);
+    std::vector<uint32_t> prog_var(Rpad, 0), flipmask(Rpad / 32, 0), prog_cls(Rpad, 0);
+    std::vector<uint8_t> prog_count_row(Rpad, 0);
+    for (uint64_t v = 0; v < V; ++v) {
+        uint64_t r = c->var_prog_off[v];
+        if (var_rows[v] & 1) {
+            prog_var[r] = (uint32_t)v;
+            prog_count_row[r] = 1;
+            ++r;
+        }
+        if (var_rows[v] & 2) {
+            prog_var[r] = (uint32_t)v;
+            flipmask[r >> 5] |= 1u << (r & 31);
+            if (!(var_rows[v] & 1)) prog_count_row[r] = 1;
+        }
+    }
+    // fill padding rows with the last real variant (never read: masked by range)
+    for (uint64_t r = R; r < Rpad; ++r) prog_var[r] = R ? prog_var[R - 1] : 0;
+    for (uint64_t v = 0; v < V; ++v) {
+        const uint64_t r = c->var_prog_off[v];
+        uint32_t m = 0;
+        for (uint64_t e = c->row_offsets[v]; e < c->row_offsets[v + 1]; ++e) {
+            const uint32_t cl = c->score_class[c->cols[e]];
+            if (cl < 32) m |= 1u << cl;
+        }
+        prog_cls[r] = m;  // first row of the variant is its count row
+    }
+
+    // ---- dense effect-allele weight rows (double), row-major [R][K]
+    std::vector<double> wrow(Rpad * K, 0.0);
+    parallel_for(V, [&](uint64_t a, uint64_t b) {
+        for (uint64_t v = a; v < b; ++v) {
+            const uint64_t r0 = c->var_prog_off[v];
+            const bool has_d = var_rows[v] & 1, has_e = var_rows[v] & 2;
+            const uint64_t rd = r0, re = has_d ? r0 + 1 : r0;
+            for (uint64_t e = c->row_offsets[v]; e < c->row_offsets[v + 1]; ++e) {
+                const double we = p->sparse_weights[e], co = p->sparse_missing_corrections[e];
+                const uint32_t k = c->cols[e];
+                if (has_d) wrow[rd * K + k] += we + 0.5 * co;
+                if (has_e) wrow[re * K + k] += 0.5 * co;
+            }
+        }
+    });
+
+    // ---- device
+    GNM_HIP(hipSetDevice(c->device));
+    GNM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    gnm_status st;
+    uint32_t *d_prog_var, *d_flipmask, *d_prog_cls;
+    if ((st = c->upload(&d_prog_var, prog_var)) != GNM_OK) return st;
+    if ((st = c->upload(&d_flipmask, flipmask)) != GNM_OK) return st;
+    if ((st = c->upload(&d_prog_cls, prog_cls)) != GNM_OK) return st;
+    if ((st = c->upload(&c->d_score_class, c->score_class)) != GNM_OK) return st;
+    DevicePlan &pl = c->plan;
+    pl.num_prog_rows = R;
+    pl.prog_var = d_prog_var;
+    pl.flipmask = d_flipmask;
+    pl.prog_cls = d_prog_cls;
+    pl.nclass = c->nclass;
+    pl.score_class = c->d_score_class;
+    pl.limbs = c->limbs;
+
+    const bool mfma_weights = c->precision != GNM_PRECISION_FP32;
+    c->fp32_count_in_kernel = c->nclass <= (uint32_t)kFp32MaxClasses;
+    const bool mfma_counts = mfma_weights || !c->fp32_count_in_kernel;
+
+    if (!mfma_weights) {
+        const uint64_t k32 = (K + kFp32ScoreChunk - 1) / kFp32ScoreChunk * kFp32ScoreChunk;
+        std::vector<float> w32(Rpad * k32, 0.0f);
+        for (uint64_t r = 0; r < R; ++r)
+            for (uint64_t k = 0; k < K; ++k) w32[r * k32 + k] = (float)wrow[r * K + k];
+        float *d_w32;
+        if ((st = c->upload(&d_w32, w32)) != GNM_OK) return st;
+        pl.w32 = d_w32;
+        pl.k32 = k32;
+    }
+    const uint32_t L = c->limbs;
+    pl.ncol = mfma_weights ? (uint32_t)(L * K) : 0;
+    pl.wtiles = (pl.ncol + 15) / 16;
+    pl.ctiles = mfma_counts ? (c->nclass + 15) / 16 : 0;
+    const uint32_t tiles = pl.wtiles + pl.ctiles;
+    if (tiles) {
+        const uint64_t nks = Rpad / kKStep;
+        const uint64_t nblk = (Rpad + kFlushRows - 1) / kFlushRows;
+        std::vector<double> scales(mfma_weights ? nblk * pl.ncol : 1, 0.0);
+        // per (scale block, score): u = 2^e with max|w|/u <= 2^(8L) - 2^(8(L-1))
+        std::vector<int> uexp(mfma_weights ? nblk * K : 1, 0);
+        if (mfma_weights) {
+            const double qmax = std::ldexp(1.0, 8 * L) - std::ldexp(1.0, 8 * (L - 1));
+            parallel_for(nblk, [&](uint64_t a, uint64_t b) {
+                for (uint64_t blk = a; blk < b; ++blk) {
+                    const uint64_t r0 = blk * kFlushRows, r1 = std::min<uint64_t>(R, r0 + kFlushRows);
+                    for (uint64_t k = 0; k < K; ++k) {
+                        double m = 0.0;
+                        for (uint64_t r = r0; r < r1; ++r) m = std::max(m, std::fabs(wrow[r * K + k]));
+                        int e = 0;
+                        if (m > 0.0) {
+                            e = (int)std::ceil(std::log2(m / qmax));
+                            while (std::ldexp(qmax, e) < m) ++e;
+                            while (std::ldexp(qmax, e - 1) >= m) --e;
+                        }
+                        uexp[blk * K + k] = e;
+                        for (uint32_t j = 0; j < L; ++j)
+                            scales[blk * pl.ncol + j * K + k] = std::ldexp(1.0, e + 8 * (int)(L - 1 - j));
+                    }
+                }
+            });
+        }
+        // B fragments: [ks][tile][lane][8 bf16]; lane (n = lane%16, g = lane/16)
+        // holds rows 32ks + 8g + j (j = 0..7) of column tile*16 + n.
+        std::vector<uint16_t> frag(nks * tiles * 64 * 8, 0);
+        parallel_for(nks, [&](uint64_t a, uint64_t b) {
+            std::vector<int32_t> dig(L);
+            for (uint64_t ks = a; ks < b; ++ks) {
+                const uint64_t blk = ks * kKStep / kFlushRows;
+                for (uint32_t tile = 0; tile < tiles; ++tile) {
+                    uint16_t *dst = frag.data() + ((ks * tiles + tile) * 64) * 8;
+                    for (int lane = 0; lane < 64; ++lane) {
+                        const uint32_t n = lane & 15, g = lane >> 4;
+                        for (int j = 0; j < 8; ++j) {
+                            const uint64_t r = ks * kKStep + g * 8 + j;
+                            uint16_t val = 0;
+                            if (r < R) {
+                                if (tile < pl.wtiles) {
+                                    const uint32_t colw = tile * 16 + n;
+                                    if (colw < pl.ncol) {
+                                        const uint32_t limb = colw / (uint32_t)K;
+                                        const uint64_t k = colw % K;
+                                        const double w = wrow[r * K + k];
+                                        if (w != 0.0) {
+                                            const double Qd = std::nearbyint(std::ldexp(w, -uexp[blk * K + k]));
+                                            limb_digits((int64_t)Qd, L, dig.data());
+                                            val = bf16_of_int(dig[limb]);
+                                        }
+                                    }
+                                } else {
+                                    const uint32_t cl = (tile - pl.wtiles) * 16 + n;
+                                    if (cl < c->nclass && prog_count_row[r]) {
+                                        const uint64_t v = prog_var[r];
+                                        for (uint64_t e = c->row_offsets[v]; e < c->row_offsets[v + 1]; ++e)
+                                            if (c->score_class[c->cols[e]] == cl) { val = 0x3F80; break; }
+                                    }
+                                }
+                            }
+                            dst[lane * 8 + j] = val;
+                        }
+                    }
+                }
+            }
+        });
+        uint16_t *d_frag;
+        if ((st = c->upload(&d_frag, frag)) != GNM_OK) return st;
+        pl.bfrag = (const uint32_t *)d_frag;
+        if (mfma_weights) {
+            double *d_scales;
+            if ((st = c->upload(&d_scales, scales)) != GNM_OK) return st;
+            pl.scales = d_scales;
+        }
+    }
+
+    // ---- accumulators and partials
+    const uint64_t cells = c->N * K;
+    if ((st = c->alloc(&c->d_scores, cells)) != GNM_OK) return st;
+    if ((st = c->alloc(&c->d_counts, cells)) != GNM_OK) return st;
+    GNM_HIP(launch_fill_f64(c->d_scores, cells, 0.0, c->stream));
+    GNM_HIP(launch_fill_u32(c->d_counts, cells, 0u, c->stream));
+    c->splits = 8;
+    const uint64_t ncol_part = mfma_weights ? (uint64_t)L * K : K;
+    if ((st = c->alloc(&c->d_part, (uint64_t)c->splits * c->N * ncol_part)) != GNM_OK) return st;
+    if ((st = c->alloc(&c->d_cpart, (uint64_t)c->splits * c->N * c->nclass)) != GNM_OK) return st;
+    if (!c->identity) {
+        std::vector<uint32_t> o2f(p->output_idx_to_fam_idx, p->output_idx_to_fam_idx + c->N);
+        if ((st = c->upload(&c->d_out_to_fam, o2f)) != GNM_OK) return st;
+        c->cbpv = (c->N + 3) / 4;
+    }
+    GNM_HIP(hipEventCreate(&c->ev_main0));
+    GNM_HIP(hipEventCreate(&c->ev_main1));
+    GNM_HIP(hipEventCreate(&c->ev_all0));
+    GNM_HIP(hipEventCreate(&c->ev_all1));
+    GNM_HIP(hipStreamSynchronize(c->stream));
+    return GNM_OK;
+}
+
+void destroy_context(gnm_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int i = 0; i < 2; ++i) {
+        if (c->stage_free[i]) (void)hipEventDestroy(c->stage_free[i]);
+        if (c->h_stage[i]) (void)hipHostFree(c->h_stage[i]);
+    }
+    for (hipEvent_t e : {c->ev_main0, c->ev_main1, c->ev_all0, c->ev_all1})
+        if (e) (void)hipEventDestroy(e);
+    for (auto it = c->allocations.rbegin(); it != c->allocations.rend(); ++it) (void)hipFree(it->ptr);
+    c->allocations.clear();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+// Sum of missing corrections of variants [from, V) per score: the part of
+// the reference baseline (score/prepare.rs:632-650) not yet accounted for.
+std::vector<double> baseline_tail(const gnm_ctx *c, uint64_t from) {
+    std::vector<double> b(c->K, 0.0);
+    for (uint64_t v = from; v < c->V; ++v)
+        for (uint64_t e = c->row_offsets[v]; e < c->row_offsets[v + 1]; ++e) b[c->cols[e]] += (double)c->corr[e];
+    return b;
+}
+
+gnm_status score_device_rows(gnm_ctx *c, const uint8_t *d_rows, uint64_t stride, uint64_t v0, uint64_t n) {
+    if (n == 0) return GNM_OK;
+    const uint8_t *rows = d_rows;
+    uint64_t rstride = stride, bpv = c->bpv;
+    GNM_HIP(hipEventRecord(c->ev_all0, c->stream));
+    if (!c->identity) {
+        if (c->compact_rows < n) {
+            // grow the compaction buffer (rare: sized by the largest block seen)
+            if (c->d_compact) {
+                GNM_HIP(hipStreamSynchronize(c->stream));
+                for (auto it = c->allocations.begin(); it != c->allocations.end(); ++it)
+                    if (it->ptr == c->d_compact) {
+                        (void)hipFree(it->ptr);
+                        c->allocations.erase(it);
+                        break;
+                    }
+                c->d_compact = nullptr;
+            }
+            gnm_status st = c->alloc(&c->d_compact, n * c->cbpv);
+            if (st != GNM_OK) return st;
+            c->compact_rows = n;
+        }
+        GNM_HIP(launch_compact(d_rows, stride, n, c->d_out_to_fam, c->N, c->d_compact, c->cbpv, c->stream));
+        rows = c->d_compact;
+        rstride = c->cbpv;
+        bpv = c->cbpv;
+    }
+    const uint64_t p0 = c->var_prog_off[v0], p1 = c->var_prog_off[v0 + n];
+    const uint32_t S = c->splits;
+    GNM_HIP(hipEventRecord(c->ev_main0, c->stream));
+    if (c->precision == GNM_PRECISION_FP32) {
+        const uint64_t rps = (p1 - p0 + S - 1) / S;
+        GNM_HIP(launch_score_fp32(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, c->K, S, rps,
+                                  c->fp32_count_in_kernel, c->d_part, c->d_cpart, c->stream));
+        if (!c->fp32_count_in_kernel) {
+            const uint64_t base = (p0 / kFlushRows) * kFlushRows;
+            uint64_t rps2 = (p1 - base + S - 1) / S;
+            rps2 = (rps2 + kFlushRows - 1) / kFlushRows * kFlushRows;
+            GNM_HIP(launch_score_mfma(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, S, rps2, 0,
+                                      c->plan.ctiles, c->d_part, c->d_cpart, c->stream));
+        }
+    } else {
+        const uint64_t base = (p0 / kFlushRows) * kFlushRows;
+        uint64_t rps = (p1 - base + S - 1) / S;
+        rps = (rps + kFlushRows - 1) / kFlushRows * kFlushRows;
+        GNM_HIP(launch_score_mfma(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, S, rps, 0,
+                                  c->plan.wtiles + c->plan.ctiles, c->d_part, c->d_cpart, c->stream));
+    }
+    GNM_HIP(hipEventRecord(c->ev_main1, c->stream));
+    GNM_HIP(launch_reduce(c->d_part, c->d_cpart, S, c->N, c->K,
+                          c->precision == GNM_PRECISION_FP32 ? 1 : c->limbs, c->nclass,
+                          c->d_score_class, c->d_scores, c->d_counts, c->stream));
+    GNM_HIP(hipEventRecord(c->ev_all1, c->stream));
+    c->timing_valid = true;
+    c->completed = v0 + n;
+    return GNM_OK;
+}
+
+gnm_status check_block(gnm_ctx *c, uint64_t first_variant, uint64_t num_rows, uint64_t row_stride) {
+    if (!c) return fail(GNM_ERR_INVALID_ARG, "ctx is NULL");
+    if (first_variant > c->V || num_rows > c->V - first_variant)
+        return fail(GNM_ERR_INVALID_ARG, "row block exceeds num_variants");
+    if (row_stride < c->bpv) return fail(GNM_ERR_INVALID_ARG, "row_stride < bytes_per_variant");
+    return GNM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gnm_last_error(void) { return g_last_error.c_str(); }
+int gnm_abi_version(void) { return GNM_ABI_VERSION; }
+
+int gnm_size_gate(uint64_t num_people, uint64_t num_scores, uint64_t num_variants) {
+    const unsigned __int128 cells = (unsigned __int128)num_people * num_scores;
+    const unsigned __int128 work = cells * num_variants;
+    return (cells >= 100000u || work >= 100000000u) ? 1 : 0;
+}
+
+int gnm_device_count(void) {
+    try {
+        if (env_hides_devices()) return 0;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+        int ok = 0;
+        for (int i = 0; i < n; ++i) {
+            hipDeviceProp_t prop;
+            if (hipGetDeviceProperties(&prop, i) != hipSuccess) continue;
+            if (std::strncmp(prop.gcnArchName, "gfx950", 6) == 0) ++ok;
+        }
+        return ok;
+    } catch (...) {
+        return 0;
+    }
+}
+
+static gnm_status create_impl(const gnm_problem *problem, gnm_ctx **out_ctx, bool gate) {
+    try {
+        if (!out_ctx) return fail(GNM_ERR_INVALID_ARG, "out_ctx is NULL");
+        *out_ctx = nullptr;
+        gnm_status st = validate(problem);
+        if (st != GNM_OK) return st;
+        if (gate && !gnm_size_gate(problem->num_people, problem->num_scores, problem->num_variants))
+            return fail(GNM_DECLINED, "problem size below the device threshold");
+        if (gnm_device_count() == 0) return fail(GNM_ERR_NO_DEVICE, "no visible gfx950 device");
+        gnm_ctx *c = new gnm_ctx();
+        st = build_context(problem, c);
+        if (st != GNM_OK) {
+            std::string msg = g_last_error;
+            destroy_context(c);
+            g_last_error = msg;
+            return st;
+        }
+        *out_ctx = c;
+        return GNM_OK;
+    } catch (const std::bad_alloc &) {
+        return fail(GNM_ERR_OUT_OF_MEMORY, "host allocation failed while building the plan");
+    } catch (const std::exception &e) {
+        return fail(GNM_ERR_INTERNAL, e.what());
+    } catch (...) {
+        return fail(GNM_ERR_INTERNAL, "unknown exception");
+    }
+}
+
+gnm_status gnm_create(const gnm_problem *problem, gnm_ctx **out_ctx) { return create_impl(problem, out_ctx, true); }
+gnm_status gnm_create_forced(const gnm_problem *problem, gnm_ctx **out_ctx) { return create_impl(problem, out_ctx, false); }
+
+gnm_status gnm_score_rows_device(gnm_ctx *ctx, const uint8_t *d_rows, uint64_t row_stride,
+                                 uint64_t first_variant, uint64_t num_rows) {
+    try {
+        gnm_status st = check_block(ctx, first_variant, num_rows, row_stride);
+        if (st != GNM_OK) return st;
+        if (num_rows && !d_rows) return fail(GNM_ERR_INVALID_ARG, "d_rows is NULL");
+        GNM_HIP(hipSetDevice(ctx->device));
+        return score_device_rows(ctx, d_rows, row_stride, first_variant, num_rows);
+    } catch (...) {
+        return fail(GNM_ERR_INTERNAL, "exception in gnm_score_rows_device");
+    }
+}
+
+gnm_status gnm_score_rows(gnm_ctx *ctx, const uint8_t *rows, uint64_t row_stride,
+                          uint64_t first_variant, uint64_t num_rows) {
+    try {
+        gnm_status st = check_block(ctx, first_variant, num_rows, row_stride);
+        if (st != GNM_OK) return st;
+        if (num_rows && !rows) return fail(GNM_ERR_INVALID_ARG, "rows is NULL");
+        GNM_HIP(hipSetDevice(ctx->device));
+        gnm_ctx *c = ctx;
+        const uint64_t bpv = c->bpv;
+        if (!c->h_stage[0]) {
+            // two pinned slots of ~256 MiB (at least one row) -- the reference's
+            // PIPELINE_SLOTS = 2 double buffer (cuda_backend.rs:42, 1137-1147)
+            c->stage_rows = std::max<uint64_t>(1, (256ull << 20) / bpv);
+            for (int i = 0; i < 2; ++i) {
+                GNM_HIP(hipHostMalloc((void **)&c->h_stage[i], c->stage_rows * bpv, hipHostMallocDefault));
+                gnm_status s2 = c->alloc(&c->d_stage[i], c->stage_rows * bpv);
+                if (s2 != GNM_OK) return s2;
+                GNM_HIP(hipEventCreateWithFlags(&c->stage_free[i], hipEventDisableTiming));
+                GNM_HIP(hipEventRecord(c->stage_free[i], c->stream));
+            }
+        }
+        uint64_t done = 0;
+        int slot = 0;
+        while (done < num_rows) {
+            const uint64_t n = std::min(c->stage_rows, num_rows - done);
+            GNM_HIP(hipEventSynchronize(c->stage_free[slot]));
+            const uint8_t *src = rows + done * row_stride;
+            if (row_stride == bpv) {
+                std::memcpy(c->h_stage[slot], src, n * bpv);
+            } else {
+                for (uint64_t r = 0; r < n; ++r) std::memcpy(c->h_stage[slot] + r * bpv, src + r * row_stride, bpv);
+            }
+            GNM_HIP(hipMemcpyAsync(c->d_stage[slot], c->h_stage[slot], n * bpv, hipMemcpyHostToDevice, c->stream));
+            st = score_device_rows(c, c->d_stage[slot], bpv, first_variant + done, n);
+            if (st != GNM_OK) return st;
+            GNM_HIP(hipEventRecord(c->stage_free[slot], c->stream));
+            done += n;
+            slot ^= 1;
+        }
+        // the caller may reuse `rows` now: every byte has been copied to pinned staging
+        return GNM_OK;
+    } catch (...) {
+        return fail(GNM_ERR_INTERNAL, "exception in gnm_score_rows");
+    }
+}
+
+gnm_status gnm_read_accumulators(gnm_ctx *ctx, double *scores, uint32_t *counts, uint64_t *completed_variants) {
+    try {
+        if (!ctx || !scores || !counts) return fail(GNM_ERR_INVALID_ARG, "NULL argument");
+        GNM_HIP(hipSetDevice(ctx->device));
+        GNM_HIP(hipStreamSynchronize(ctx->stream));
+        const uint64_t cells = ctx->N * ctx->K;
+        GNM_HIP(hipMemcpy(scores, ctx->d_scores, cells * sizeof(double), hipMemcpyDeviceToHost));
+        GNM_HIP(hipMemcpy(counts, ctx->d_counts, cells * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        const std::vector<double> tail = baseline_tail(ctx, ctx->completed);
+        for (uint64_t p = 0; p < ctx->N; ++p)
+            for (uint64_t k = 0; k < ctx->K; ++k) scores[p * ctx->K + k] += tail[k];
+        if (completed_variants) *completed_variants = ctx->completed;
+        return GNM_OK;
+    } catch (...) {
+        return fail(GNM_ERR_INTERNAL, "exception in gnm_read_accumulators");
+    }
+}
+
+gnm_status gnm_seed_accumulators(gnm_ctx *ctx, const double *scores, const uint32_t *counts, uint64_t completed_variants) {
+    try {
+        if (!ctx || !scores || !counts) return fail(GNM_ERR_INVALID_ARG, "NULL argument");
+        if (completed_variants > ctx->V) return fail(GNM_ERR_INVALID_ARG, "completed_variants > num_variants");
+        GNM_HIP(hipSetDevice(ctx->device));
+        GNM_HIP(hipStreamSynchronize(ctx->stream));
+        const uint64_t cells = ctx->N * ctx->K;
+        const std::vector<double> tail = baseline_tail(ctx, completed_variants);
+        std::vector<double> s(scores, scores + cells);
+        for (uint64_t p = 0; p < ctx->N; ++p)
+            for (uint64_t k = 0; k < ctx->K; ++k) s[p * ctx->K + k] -= tail[k];
+        GNM_HIP(hipMemcpy(ctx->d_scores, s.data(), cells * sizeof(double), hipMemcpyHostToDevice));
+        GNM_HIP(hipMemcpy(ctx->d_counts, counts, cells * sizeof(uint32_t), hipMemcpyHostToDevice));
+        ctx->completed = completed_variants;
+        return GNM_OK;
+    } catch (...) {
+        return fail(GNM_ERR_INTERNAL, "exception in gnm_seed_accumulators");
+    }
+}
+
+gnm_status gnm_finish(gnm_ctx *ctx, double *scores, uint32_t *counts) {
+    uint64_t done = 0;
+    gnm_status st = gnm_read_accumulators(ctx, scores, counts, &done);
+    if (st != GNM_OK) return st;
+    if (done != ctx->V) return fail(GNM_ERR_INVALID_ARG, "gnm_finish before all variants were scored");
+    return GNM_OK;
+}
+
+void gnm_destroy(gnm_ctx *ctx) {
+    try {
+        destroy_context(ctx);
+    } catch (...) {
+    }
+}
+
+gnm_status gnm_run(const gnm_problem *problem, const uint8_t *rows, uint64_t row_stride, double *scores,
+                   uint32_t *counts) {
+    gnm_ctx *c = nullptr;
+    gnm_status st = gnm_create(problem, &c);
+    if (st != GNM_OK) return st;
+    st = gnm_score_rows(c, rows, row_stride, 0, problem->num_variants);
+    if (st == GNM_OK) st = gnm_finish(c, scores, counts);
+    std::string msg = g_last_error;
+    gnm_destroy(c);
+    g_last_error = msg;
+    return st;
+}
+
+gnm_status gnm_last_timing(gnm_ctx *ctx, float *main_kernel_ms, float *total_ms) {
+    try {
+        if (!ctx || !ctx->timing_valid) return fail(GNM_ERR_INVALID_ARG, "no timed call on this context");
+        GNM_HIP(hipSetDevice(ctx->device));
+        GNM_HIP(hipEventSynchronize(ctx->ev_all1));
+        float a = 0.f, b = 0.f;
+        GNM_HIP(hipEventElapsedTime(&a, ctx->ev_main0, ctx->ev_main1));
+        GNM_HIP(hipEventElapsedTime(&b, ctx->ev_all0, ctx->ev_all1));
+        if (main_kernel_ms) *main_kernel_ms = a;
+        if (total_ms) *total_ms = b;
+        return GNM_OK;
+    } catch (...) {
+        return fail(GNM_ERR_INTERNAL, "exception in gnm_last_timing");
+    }
+}
+
+gnm_status gnm_unpack_device(const uint8_t *d_rows, uint64_t bytes_per_variant, uint64_t num_variants,
+                             uint64_t num_people, const uint8_t *d_flip, uint8_t *d_dosage,
+                             uint32_t *d_missing_per_person, void *stream) {
+    try {
+        if (num_variants && (!d_rows || !d_dosage || !d_missing_per_person))
+            return fail(GNM_ERR_INVALID_ARG, "NULL device pointer");
+        if (bytes_per_variant < (num_people + 3) / 4) return fail(GNM_ERR_INVALID_ARG, "bytes_per_variant too small");
+        GNM_HIP(launch_unpack(d_rows, bytes_per_variant, num_variants, num_people, d_flip, d_dosage,
+                              d_missing_per_person, (hipStream_t)stream));
+        return GNM_OK;
+    } catch (...) {
+        return fail(GNM_ERR_INTERNAL, "exception in gnm_unpack_device");
+    }
+}
+
+gnm_status gnm_synth_rows_device(uint64_t seed, uint32_t p_lo24, uint32_t p_hi24, uint32_t miss16,
+                                 uint64_t num_samples, uint64_t first_variant, uint64_t num_rows,
+                                 uint8_t *d_out, uint64_t row_stride, void *stream) {
+    try {
+        if (num_rows && !d_out) return fail(GNM_ERR_INVALID_ARG, "d_out is NULL");
+        if (p_hi24 < p_lo24 || p_hi24 > (1u << 24)) return fail(GNM_ERR_INVALID_ARG, "bad frequency range");
+        if (miss16 > 65536) return fail(GNM_ERR_INVALID_ARG, "bad missing rate");
+        if (row_stride < (num_samples + 3) / 4) return fail(GNM_ERR_INVALID_ARG, "row_stride too small");
+        GNM_HIP(launch_synth(seed, p_lo24, p_hi24, miss16, num_samples, first_variant, num_rows, d_out,
+                             row_stride, (hipStream_t)stream));
+        return GNM_OK;
+    } catch (...) {
+        return fail(GNM_ERR_INTERNAL, "exception in gnm_synth_rows_device");
+    }
+}
+
+gnm_status gnm_dense_forward_device(const uint8_t *, uint64_t, uint64_t, uint64_t, const uint8_t *,
+                                    const float *, uint64_t, int32_t, double *, uint32_t *, void *) {
+    return fail(GNM_ERR_UNSUPPORTED, "gnm_dense_forward_device: not built yet");
+}
+
+gnm_status gnm_dense_adjoint_device(const uint8_t *, uint64_t, uint64_t, uint64_t, const uint8_t *,
+                                    const float *, uint64_t, int32_t, float *, void *) {
+    return fail(GNM_ERR_UNSUPPORTED, "gnm_dense_adjoint_device: not built yet");
+}
+
+}  // extern "C"

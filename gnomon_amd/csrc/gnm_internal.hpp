@@ -1,0 +1,75 @@
+// Internal declarations shared by the HIP kernels (gnm_kernels.hip) and the
+// host-side plan/context code (gnm_api.hip).  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gnm {
+
+// Program rows are grouped in k-steps of 32 (the reduction depth of one
+// v_mfma_f32_16x16x32_bf16) and in scale blocks of kFlushRows (the span over
+// which FP32 partial sums of integer-valued BF16 limbs stay exact:
+// 8192 rows * dosage 2 * |limb| 256 = 2^22 < 2^24).
+constexpr int kKStep = 32;
+constexpr int kFlushRows = 8192;
+constexpr int kKStepsPerFlush = kFlushRows / kKStep;
+constexpr int kLimbBits = 8;
+// The FP32 path flushes its FP32 accumulators to FP64 every 256 variants, the
+// reference kernel's mini-batch (score/batch.rs:34).
+constexpr int kFp32FlushRows = 256;
+constexpr int kFp32ScoreChunk = 8;     // scores per pass of the FP32 kernel
+constexpr int kFp32MaxClasses = 4;     // membership classes counted in-kernel by the FP32 kernel
+constexpr int kMfmaMaxTiles = 12;      // 16-column tiles per pass of the MFMA kernel
+constexpr int kPersonsPerMfmaBlock = 256;
+constexpr int kPersonsPerFp32Block = 1024;
+
+// Device-side view of the weight plan built by gnm_create.
+struct DevicePlan {
+    uint64_t num_prog_rows;    // R: reduction rows (one per variant, two for mixed-flip variants)
+    const uint32_t *prog_var;  // [R] reconciled variant of each row
+    const uint32_t *flipmask;  // [ceil(R/32)] bit j of word s: row 32s+j reads the A1 allele
+    const uint32_t *prog_cls;  // [R] membership-class bitmask of count rows (FP32 kernel, <=32 classes)
+    // FP32 path
+    const float *w32;          // [R][k32] effect-allele weights (k32 = K rounded up to 8)
+    uint64_t k32;
+    // split-BF16 path
+    const uint32_t *bfrag;     // [R/32][tiles][64 lanes][4 dwords] B fragments (weights then counts)
+    const double *scales;      // [R/kFlushRows][ncol] power-of-two limb scales
+    uint32_t limbs;            // 2 or 3
+    uint32_t ncol;             // limbs * K (weight columns, column = limb*K + score)
+    uint32_t wtiles;           // ceil(ncol/16)
+    uint32_t ctiles;           // ceil(nclass/16)
+    uint32_t nclass;           // membership classes (distinct CSR column supports)
+    const uint32_t *score_class; // [K]
+};
+
+// Kernel launchers (gnm_kernels.hip).  All are asynchronous on `stream`.
+hipError_t launch_synth(uint64_t seed, uint32_t p_lo24, uint32_t p_hi24, uint32_t miss16,
+                        uint64_t num_samples, uint64_t first_variant, uint64_t num_rows,
+                        uint8_t *out, uint64_t row_stride, hipStream_t stream);
+hipError_t launch_unpack(const uint8_t *rows, uint64_t stride, uint64_t num_rows,
+                         uint64_t num_people, const uint8_t *flip, uint8_t *dosage,
+                         uint32_t *missing, hipStream_t stream);
+hipError_t launch_compact(const uint8_t *rows, uint64_t stride, uint64_t num_rows,
+                          const uint32_t *out_to_fam, uint64_t num_people, uint8_t *out,
+                          uint64_t out_stride, hipStream_t stream);
+hipError_t launch_score_fp32(const DevicePlan &plan, const uint8_t *rows, uint64_t stride,
+                             uint64_t bpv, uint64_t v0, uint64_t p0, uint64_t p1,
+                             uint64_t num_people, uint64_t num_scores, uint32_t splits,
+                             uint64_t rows_per_split, bool count_in_kernel, double *part,
+                             uint32_t *cpart, hipStream_t stream);
+hipError_t launch_score_mfma(const DevicePlan &plan, const uint8_t *rows, uint64_t stride,
+                             uint64_t bpv, uint64_t v0, uint64_t p0, uint64_t p1,
+                             uint64_t num_people, uint32_t splits, uint64_t rows_per_split,
+                             uint32_t tile_begin, uint32_t tile_end, double *part,
+                             uint32_t *cpart, hipStream_t stream);
+// scores[p][k] += sum_s sum_l part[s][p][l*K + k]; counts[p][k] += sum_s cpart[s][p][class[k]]
+hipError_t launch_reduce(const double *part, const uint32_t *cpart, uint32_t splits,
+                         uint64_t num_people, uint64_t num_scores, uint32_t limbs,
+                         uint32_t nclass, const uint32_t *score_class, double *scores,
+                         uint32_t *counts, hipStream_t stream);
+hipError_t launch_fill_f64(double *p, uint64_t n, double value, hipStream_t stream);
+hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t value, hipStream_t stream);
+
+}  // namespace gnm

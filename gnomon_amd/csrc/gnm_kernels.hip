@@ -1,0 +1,543 @@
+// gfx950 kernels of the polygenic-scoring hot path.
+//
+// Genotype decode follows the PLINK SNP-major layout used by the reference
+// (score/batch.rs:489-503, 797-844): sample j of a row sits in byte j>>2 at
+// bits 2(j&3); codes 00 -> A2 dosage 0, 10 -> 1, 11 -> 2, 01 -> missing.
+// Every scoring kernel works in effect-allele space: a row flagged "flip"
+// reads the A1 allele, whose dosage is 2-d; the flip is the code map
+// 00<->11 (01 and 10 fixed), i.e. swap the two bits of each field and invert.
+// A missing genotype always contributes 0, which reproduces the reference's
+// baseline-plus-missing-correction arithmetic (score/prepare.rs:197-208,
+// score/batch.rs:372-385) without a separate correction contraction.
+#include "gnm_internal.hpp"
+
+namespace gnm {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Effect-allele flip of four packed genotypes: 00<->11, 01 and 10 unchanged.
+__device__ __forceinline__ uint32_t flip_byte(uint32_t b) {
+    return (~(((b & 0x55u) << 1) | ((b >> 1) & 0x55u))) & 0xFFu;
+}
+
+// ---------------------------------------------------------------------------
+// Synthetic rows (counter-based hash, integer-only: identical to the CPU
+// generator of the test oracle).
+// ---------------------------------------------------------------------------
+__global__ void k_synth(uint64_t seed, uint32_t p_lo24, uint32_t p_hi24, uint32_t miss16,
+                        uint64_t num_samples, uint64_t first_variant, uint64_t num_rows,
+                        uint64_t bpv, uint8_t *__restrict__ out, uint64_t row_stride) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= bpv) return;
+    for (uint64_t r = blockIdx.y; r < num_rows; r += gridDim.y) {
+        const uint64_t v = first_variant + r;
+        const uint64_t kv = mix64(seed ^ mix64(v + 0x632BE59BD9B4E019ull));
+        const uint64_t u24 = mix64(kv ^ 0xA0761D6478BD642Full) >> 40;
+        const uint32_t p24 = p_lo24 + (uint32_t)(((uint64_t)(p_hi24 - p_lo24) * u24) >> 24);
+        const uint64_t q24 = (1ull << 24) - p24;
+        const uint64_t t0 = q24 * q24;
+        const uint64_t t1 = t0 + 2ull * (uint64_t)p24 * q24;
+        uint32_t byte = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint64_t s = b * 4 + j;
+            if (s < num_samples) {
+                const uint64_t h = mix64(kv + (s + 1) * 0x9E3779B97F4A7C15ull);
+                uint32_t code;
+                if ((uint32_t)(h >> 48) < miss16) code = 1u;
+                else {
+                    const uint64_t u48 = h & 0xFFFFFFFFFFFFull;
+                    code = (u48 < t0) ? 0u : (u48 < t1 ? 2u : 3u);
+                }
+                byte |= code << (2 * j);
+            }
+        }
+        out[r * row_stride + b] = (uint8_t)byte;
+    }
+}
+
+hipError_t launch_synth(uint64_t seed, uint32_t p_lo24, uint32_t p_hi24, uint32_t miss16,
+                        uint64_t num_samples, uint64_t first_variant, uint64_t num_rows,
+                        uint8_t *out, uint64_t row_stride, hipStream_t stream) {
+    const uint64_t bpv = (num_samples + 3) / 4;
+    if (num_rows == 0 || bpv == 0) return hipSuccess;
+    dim3 grid((unsigned)((bpv + 255) / 256), (unsigned)(num_rows < 16384 ? num_rows : 16384));
+    hipLaunchKernelGGL(k_synth, grid, dim3(256), 0, stream, seed, p_lo24, p_hi24, miss16,
+                       num_samples, first_variant, num_rows, bpv, out, row_stride);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Standalone unpack: one thread per packed byte column (4 people) and a
+// chunk of rows; writes the reference tile values (0,1,2 effect dosage,
+// 3 missing) and accumulates per-person missing counts.
+// ---------------------------------------------------------------------------
+constexpr int kUnpackRowChunk = 256;
+
+__global__ __launch_bounds__(256) void k_unpack(const uint8_t *__restrict__ rows, uint64_t stride,
+                                                uint64_t num_rows, uint64_t num_people,
+                                                const uint8_t *__restrict__ flip,
+                                                uint8_t *__restrict__ dosage,
+                                                uint32_t *__restrict__ missing) {
+    const uint64_t col = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t bpv = (num_people + 3) / 4;
+    if (col >= bpv) return;
+    const uint64_t r0 = (uint64_t)blockIdx.y * kUnpackRowChunk;
+    const uint64_t r1 = r0 + kUnpackRowChunk < num_rows ? r0 + kUnpackRowChunk : num_rows;
+    const uint64_t p0 = col * 4;
+    const bool full = (p0 + 4 <= num_people) && ((num_people & 3) == 0);
+    uint32_t miss = 0;  // four 8-bit counters (chunk <= 255 rows per flush)
+    uint32_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+    for (uint64_t r = r0; r < r1; ++r) {
+        uint32_t b = rows[r * stride + col];
+        if (flip && flip[r]) b = flip_byte(b);
+        // spread the four 2-bit codes into four bytes, then map code -> tile value
+        const uint32_t x = (b & 0x03u) | ((b & 0x0Cu) << 6) | ((b & 0x30u) << 12) | ((b & 0xC0u) << 18);
+        const uint32_t t = __builtin_amdgcn_perm(0u, 0x02010300u, x);
+        // missing flag per byte: tile value 3
+        const uint32_t is3 = t & (t >> 1) & 0x01010101u;
+        miss += is3;
+        if ((r - r0) == 254) {  // drain byte counters before they can wrap
+            m0 += miss & 0xFF; m1 += (miss >> 8) & 0xFF; m2 += (miss >> 16) & 0xFF; m3 += miss >> 24;
+            miss = 0;
+        }
+        uint8_t *dst = dosage + r * num_people + p0;
+        if (full) {
+            *(uint32_t *)dst = t;
+        } else {
+            for (int j = 0; j < 4; ++j)
+                if (p0 + j < num_people) dst[j] = (uint8_t)(t >> (8 * j));
+        }
+    }
+    m0 += miss & 0xFF; m1 += (miss >> 8) & 0xFF; m2 += (miss >> 16) & 0xFF; m3 += miss >> 24;
+    if (m0 && p0 + 0 < num_people) atomicAdd(&missing[p0 + 0], m0);
+    if (m1 && p0 + 1 < num_people) atomicAdd(&missing[p0 + 1], m1);
+    if (m2 && p0 + 2 < num_people) atomicAdd(&missing[p0 + 2], m2);
+    if (m3 && p0 + 3 < num_people) atomicAdd(&missing[p0 + 3], m3);
+}
+
+hipError_t launch_unpack(const uint8_t *rows, uint64_t stride, uint64_t num_rows,
+                         uint64_t num_people, const uint8_t *flip, uint8_t *dosage,
+                         uint32_t *missing, hipStream_t stream) {
+    const uint64_t bpv = (num_people + 3) / 4;
+    if (num_rows == 0 || bpv == 0) return hipSuccess;
+    dim3 grid((unsigned)((bpv + 255) / 256), (unsigned)((num_rows + kUnpackRowChunk - 1) / kUnpackRowChunk));
+    hipLaunchKernelGGL(k_unpack, grid, dim3(256), 0, stream, rows, stride, num_rows, num_people,
+                       flip, dosage, missing);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Person-subset compaction (the --keep gather of output_idx_to_fam_idx,
+// score/types.rs:151-158): output byte j of a row packs the codes of output
+// people 4j..4j+3.
+// ---------------------------------------------------------------------------
+__global__ void k_compact(const uint8_t *__restrict__ rows, uint64_t stride, uint64_t num_rows,
+                          const uint32_t *__restrict__ out_to_fam, uint64_t num_people,
+                          uint8_t *__restrict__ out, uint64_t out_stride) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t obpv = (num_people + 3) / 4;
+    if (j >= obpv) return;
+    uint32_t fam[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) fam[t] = (j * 4 + t < num_people) ? out_to_fam[j * 4 + t] : 0xFFFFFFFFu;
+    for (uint64_t r = blockIdx.y; r < num_rows; r += gridDim.y) {
+        const uint8_t *row = rows + r * stride;
+        uint32_t byte = 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t code = (fam[t] != 0xFFFFFFFFu) ? (row[fam[t] >> 2] >> (2 * (fam[t] & 3))) & 3u : 1u;
+            byte |= code << (2 * t);
+        }
+        out[r * out_stride + j] = (uint8_t)byte;
+    }
+}
+
+hipError_t launch_compact(const uint8_t *rows, uint64_t stride, uint64_t num_rows,
+                          const uint32_t *out_to_fam, uint64_t num_people, uint8_t *out,
+                          uint64_t out_stride, hipStream_t stream) {
+    const uint64_t obpv = (num_people + 3) / 4;
+    if (num_rows == 0 || obpv == 0) return hipSuccess;
+    dim3 grid((unsigned)((obpv + 255) / 256), (unsigned)(num_rows < 8192 ? num_rows : 8192));
+    hipLaunchKernelGGL(k_compact, grid, dim3(256), 0, stream, rows, stride, num_rows, out_to_fam,
+                       num_people, out, out_stride);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Plain FP32 path: a wavefront reduction with people on lanes (four per lane,
+// one packed byte) and the row's weights broadcast from scalar registers.
+// Per genotype and score: acc = fma(dosage, w, acc) -- bit-identical to the
+// reference kernel's add (dosage 1) and fma(w, 2, acc) (dosage 2)
+// (score/kernel.rs:34-44, 143-173).  FP32 accumulators are flushed to FP64
+// every 256 rows (score/batch.rs:34, 387-419).
+// ---------------------------------------------------------------------------
+template <int KC>
+__global__ __launch_bounds__(256) void k_score_fp32(DevicePlan plan, const uint8_t *__restrict__ rows,
+                                                    uint64_t stride, uint64_t bpv, uint64_t v0,
+                                                    uint64_t p0, uint64_t p1, uint64_t split_base,
+                                                    uint64_t rps, uint64_t num_people,
+                                                    uint64_t num_scores, int count_in_kernel,
+                                                    double *__restrict__ part,
+                                                    uint32_t *__restrict__ cpart) {
+    const uint32_t s = blockIdx.y;
+    const uint32_t kc = blockIdx.z;
+    const uint64_t col = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool colok = col < bpv;
+    const uint64_t sb = split_base + (uint64_t)s * rps;
+    const uint64_t r_begin = p0 > sb ? p0 : sb;
+    const uint64_t se = sb + rps;
+    const uint64_t r_end = p1 < se ? p1 : se;
+    const bool do_count = count_in_kernel && kc == 0;
+    const float *__restrict__ wbase = plan.w32 + (uint64_t)kc * KC;
+
+    double acc64[4][KC];
+    uint32_t cnt[4][kFp32MaxClasses];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+        for (int k = 0; k < KC; ++k) acc64[t][k] = 0.0;
+#pragma unroll
+        for (int c = 0; c < kFp32MaxClasses; ++c) cnt[t][c] = 0;
+    }
+    for (uint64_t r0 = r_begin; r0 < r_end; r0 += kFp32FlushRows) {
+        const uint64_t r1 = (r0 + kFp32FlushRows < r_end) ? r0 + kFp32FlushRows : r_end;
+        float acc[4][KC];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int k = 0; k < KC; ++k) acc[t][k] = 0.0f;
+        for (uint64_t r = r0; r < r1; ++r) {
+            const uint64_t var = (uint64_t)(plan.prog_var[r] - (uint32_t)v0);
+            const uint32_t flip = (plan.flipmask[r >> 5] >> (r & 31)) & 1u;
+            uint32_t b = colok ? rows[var * stride + col] : 0x55u;
+            if (flip) b = flip_byte(b);
+            const float *__restrict__ w = wbase + r * plan.k32;
+            float wk[KC];
+#pragma unroll
+            for (int k = 0; k < KC; ++k) wk[k] = w[k];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const uint32_t c = (b >> (2 * t)) & 3u;
+                const float d = (float)((c >> 1) * ((c & 1u) + 1u));
+#pragma unroll
+                for (int k = 0; k < KC; ++k) acc[t][k] = __builtin_fmaf(d, wk[k], acc[t][k]);
+            }
+            if (do_count) {
+                const uint32_t cls = plan.prog_cls[r];
+                if (cls) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const uint32_t m = (((b >> (2 * t)) & 3u) == 1u) ? 1u : 0u;
+#pragma unroll
+                        for (int c = 0; c < kFp32MaxClasses; ++c) cnt[t][c] += m & (cls >> c);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int k = 0; k < KC; ++k) acc64[t][k] += (double)acc[t][k];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const uint64_t p = col * 4 + t;
+        if (p >= num_people) continue;
+        double *dst = part + ((uint64_t)s * num_people + p) * num_scores + (uint64_t)kc * KC;
+#pragma unroll
+        for (int k = 0; k < KC; ++k)
+            if ((uint64_t)kc * KC + k < num_scores) dst[k] = acc64[t][k];
+        if (do_count) {
+            uint32_t *cd = cpart + ((uint64_t)s * num_people + p) * plan.nclass;
+#pragma unroll
+            for (int c = 0; c < kFp32MaxClasses; ++c)
+                if ((uint32_t)c < plan.nclass) cd[c] = cnt[t][c];
+        }
+    }
+}
+
+hipError_t launch_score_fp32(const DevicePlan &plan, const uint8_t *rows, uint64_t stride,
+                             uint64_t bpv, uint64_t v0, uint64_t p0, uint64_t p1,
+                             uint64_t num_people, uint64_t num_scores, uint32_t splits,
+                             uint64_t rows_per_split, bool count_in_kernel, double *part,
+                             uint32_t *cpart, hipStream_t stream) {
+    const uint64_t cols = (num_people + 3) / 4;
+    const uint64_t split_base = p0;
+    dim3 grid((unsigned)((cols + 255) / 256), splits,
+              (unsigned)((num_scores + kFp32ScoreChunk - 1) / kFp32ScoreChunk));
+    hipLaunchKernelGGL(k_score_fp32<kFp32ScoreChunk>, grid, dim3(256), 0, stream, plan, rows,
+                       stride, bpv, v0, p0, p1, split_base, rows_per_split, num_people,
+                       num_scores, count_in_kernel ? 1 : 0, part, cpart);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Split-BF16 MFMA path (v_mfma_f32_16x16x32_bf16), genotype unpack fused.
+//
+// Workgroup: 4 waves x 64 people.  Lane (i = lane%16, g = lane/16) owns the
+// packed byte column i of its wave (people 4i..4i+3) and the eight rows
+// 8g..8g+7 of each 32-row k-step.  The four people of the byte are four
+// M-tiles (t = 0..3): M-tile t, row i is person wave_base + 4i + t.
+// Decode per pair of rows: P = b0 | b1<<16, p = P*0x0101 = [b0,b0,b1,b1];
+// selector x = ((p >> 2t) & 0x03030303) | 0x04000400 = [c0, c0+4, c1, c1+4];
+// v_perm over the 8-byte table {lo bytes 00 00 80 00 | hi bytes 00 00 3F 40}
+// yields BF16 {0, 0, 1.0, 2.0} for codes {00, 01, 10, 11}; the table
+// {00 80 00 00 | 00 3F 00 00} yields the missing indicator for counting.
+// B fragments are pre-arranged on the host in exactly the lane order of the
+// instruction, so every lane issues one 16-byte load per 16-column tile.
+// Integer-valued limbs make each FP32 partial sum exact inside a scale block
+// of kFlushRows rows; blocks are scaled and combined in FP64.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t dose_pair(uint32_t x) {
+    return __builtin_amdgcn_perm(0x403F0000u, 0x00800000u, x);
+}
+__device__ __forceinline__ uint32_t miss_pair(uint32_t x) {
+    return __builtin_amdgcn_perm(0x00003F00u, 0x00008000u, x);
+}
+
+union Frag {
+    bf16x8_t v;
+    uint32_t u[4];
+};
+
+template <int TMAX>
+__global__ __launch_bounds__(256, 2) void k_score_mfma(DevicePlan plan, const uint8_t *__restrict__ rows,
+                                                       uint64_t stride, uint64_t bpv, uint64_t v0,
+                                                       uint64_t p0, uint64_t p1, uint64_t split_base,
+                                                       uint64_t rps, uint32_t splits,
+                                                       uint32_t person_tiles, uint64_t num_people,
+                                                       uint32_t tile_begin, uint32_t ntile,
+                                                       double *__restrict__ part,
+                                                       uint32_t *__restrict__ cpart) {
+    // XCD-aware placement: workgroups are dispatched round-robin over the 8
+    // XCDs; every workgroup of one split lands on the same XCD so the split's
+    // B fragments are shared through that XCD's L2.
+    const uint32_t L = blockIdx.x;
+    const uint32_t xcd = L & 7u, slot = L >> 3;
+    const uint32_t s = xcd + 8u * (slot / person_tiles);
+    const uint32_t ptile = slot % person_tiles;
+    if (s >= splits) return;
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    const uint64_t pw0 = (uint64_t)ptile * kPersonsPerMfmaBlock + (uint64_t)wave * 64;
+    const uint64_t col = pw0 / 4 + li;
+    const bool colok = col < bpv;
+    const uint64_t sb = split_base + (uint64_t)s * rps;
+    const uint64_t r_begin = p0 > sb ? p0 : sb;
+    const uint64_t se = sb + rps;
+    const uint64_t r_end = p1 < se ? p1 : se;
+    const uint32_t wt = plan.wtiles;
+    const uint32_t tiles_total = plan.wtiles + plan.ctiles;
+    const uint32_t nw = tile_begin < wt ? ((wt - tile_begin) < ntile ? (wt - tile_begin) : ntile) : 0;
+    const bool need_miss = nw < ntile;
+
+    f32x4_t acc[4][TMAX];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int j = 0; j < TMAX; ++j) acc[t][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+
+    bool first = true;
+    uint64_t ks = r_begin / kKStep;
+    const uint64_t ks_end = (r_end + kKStep - 1) / kKStep;
+    while (ks < ks_end) {
+        const uint64_t blk = ks / kKStepsPerFlush;
+        const uint64_t ks_stop = ((blk + 1) * kKStepsPerFlush < ks_end) ? (blk + 1) * kKStepsPerFlush : ks_end;
+        for (; ks < ks_stop; ++ks) {
+            const uint32_t fm = plan.flipmask[ks];
+            uint32_t b[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint64_t r = ks * kKStep + lg * 8 + j;
+                const bool ok = (r >= r_begin) && (r < r_end) && colok;
+                uint32_t x = 0x55u;  // four missing calls: contributes nothing
+                if (ok) {
+                    const uint64_t var = (uint64_t)(plan.prog_var[r] - (uint32_t)v0);
+                    x = rows[var * stride + col];
+                }
+                if ((fm >> (lg * 8 + j)) & 1u) x = flip_byte(x);
+                b[j] = x;
+            }
+            Frag A[4], M[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t pp = (b[2 * q] | (b[2 * q + 1] << 16)) * 0x0101u;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const uint32_t x = ((pp >> (2 * t)) & 0x03030303u) | 0x04000400u;
+                    A[t].u[q] = dose_pair(x);
+                    if (need_miss) M[t].u[q] = miss_pair(x);
+                }
+            }
+            const u32x4_t *bf = (const u32x4_t *)plan.bfrag + ((uint64_t)ks * tiles_total + tile_begin) * 64 + lane;
+#pragma unroll
+            for (int j = 0; j < TMAX; ++j) {
+                if ((uint32_t)j < ntile) {
+                    Frag B;
+                    const u32x4_t bv = bf[(uint64_t)j * 64];
+                    B.u[0] = bv[0]; B.u[1] = bv[1]; B.u[2] = bv[2]; B.u[3] = bv[3];
+                    if ((uint32_t)j < nw) {
+#pragma unroll
+                        for (int t = 0; t < 4; ++t)
+                            acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[t].v, B.v, acc[t][j], 0, 0, 0);
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < 4; ++t)
+                            acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(M[t].v, B.v, acc[t][j], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        // flush this scale block: FP64 for weight columns, u32 for counts
+#pragma unroll
+        for (int j = 0; j < TMAX; ++j) {
+            if ((uint32_t)j >= ntile) continue;
+            const uint32_t tile = tile_begin + j;
+            if (tile < wt) {
+                const uint32_t c = tile * 16 + li;
+                const double sc = (c < plan.ncol) ? plan.scales[blk * plan.ncol + c] : 0.0;
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const uint64_t p = pw0 + 16 * lg + 4 * q + t;
+                        if (p < num_people && c < plan.ncol) {
+                            const double v = (double)acc[t][j][q] * sc;
+                            double *dst = part + ((uint64_t)s * num_people + p) * plan.ncol + c;
+                            *dst = first ? v : *dst + v;
+                        }
+                    }
+            } else {
+                const uint32_t c = (tile - wt) * 16 + li;
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const uint64_t p = pw0 + 16 * lg + 4 * q + t;
+                        if (p < num_people && c < plan.nclass) {
+                            const uint32_t v = (uint32_t)acc[t][j][q];
+                            uint32_t *dst = cpart + ((uint64_t)s * num_people + p) * plan.nclass + c;
+                            *dst = first ? v : *dst + v;
+                        }
+                    }
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+        }
+        first = false;
+    }
+    if (first) {  // empty split: its partial slots still have to read zero
+#pragma unroll
+        for (int j = 0; j < TMAX; ++j) {
+            if ((uint32_t)j >= ntile) continue;
+            const uint32_t tile = tile_begin + j;
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint64_t p = pw0 + 16 * lg + 4 * q + t;
+                    if (p >= num_people) continue;
+                    if (tile < wt) {
+                        const uint32_t c = tile * 16 + li;
+                        if (c < plan.ncol) part[((uint64_t)s * num_people + p) * plan.ncol + c] = 0.0;
+                    } else {
+                        const uint32_t c = (tile - wt) * 16 + li;
+                        if (c < plan.nclass) cpart[((uint64_t)s * num_people + p) * plan.nclass + c] = 0u;
+                    }
+                }
+        }
+    }
+}
+
+hipError_t launch_score_mfma(const DevicePlan &plan, const uint8_t *rows, uint64_t stride,
+                             uint64_t bpv, uint64_t v0, uint64_t p0, uint64_t p1,
+                             uint64_t num_people, uint32_t splits, uint64_t rows_per_split,
+                             uint32_t tile_begin, uint32_t tile_end, double *part,
+                             uint32_t *cpart, hipStream_t stream) {
+    const uint32_t person_tiles = (uint32_t)((num_people + kPersonsPerMfmaBlock - 1) / kPersonsPerMfmaBlock);
+    const uint64_t split_base = (p0 / kFlushRows) * kFlushRows;
+    for (uint32_t tb = tile_begin; tb < tile_end; tb += kMfmaMaxTiles) {
+        const uint32_t nt = (tile_end - tb) < (uint32_t)kMfmaMaxTiles ? (tile_end - tb) : (uint32_t)kMfmaMaxTiles;
+        const uint64_t wgs = (uint64_t)person_tiles * splits;  // splits is a multiple of 8
+        if (nt <= 4) {
+            hipLaunchKernelGGL(k_score_mfma<4>, dim3((unsigned)wgs), dim3(256), 0, stream, plan, rows,
+                               stride, bpv, v0, p0, p1, split_base, rows_per_split, splits,
+                               person_tiles, num_people, tb, nt, part, cpart);
+        } else {
+            hipLaunchKernelGGL(k_score_mfma<kMfmaMaxTiles>, dim3((unsigned)wgs), dim3(256), 0, stream,
+                               plan, rows, stride, bpv, v0, p0, p1, split_base, rows_per_split,
+                               splits, person_tiles, num_people, tb, nt, part, cpart);
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// Cross-split FP64 reduction into the persistent accumulators (fixed order,
+// deterministic), and the per-score count gather from membership classes.
+// ---------------------------------------------------------------------------
+__global__ void k_reduce(const double *__restrict__ part, const uint32_t *__restrict__ cpart,
+                         uint32_t splits, uint64_t num_people, uint64_t num_scores, uint32_t limbs,
+                         uint32_t nclass, const uint32_t *__restrict__ score_class,
+                         double *__restrict__ scores, uint32_t *__restrict__ counts) {
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= num_people * num_scores) return;
+    const uint64_t p = idx / num_scores, k = idx % num_scores;
+    const uint64_t ncol = (uint64_t)limbs * num_scores;
+    double acc = 0.0;
+    uint32_t cnt = 0;
+    const uint32_t cls = score_class[k];
+    for (uint32_t s = 0; s < splits; ++s) {
+        const double *ps = part + ((uint64_t)s * num_people + p) * ncol + k;
+        for (uint32_t l = 0; l < limbs; ++l) acc += ps[(uint64_t)l * num_scores];
+        cnt += cpart[((uint64_t)s * num_people + p) * nclass + cls];
+    }
+    scores[idx] += acc;
+    counts[idx] += cnt;
+}
+
+hipError_t launch_reduce(const double *part, const uint32_t *cpart, uint32_t splits,
+                         uint64_t num_people, uint64_t num_scores, uint32_t limbs,
+                         uint32_t nclass, const uint32_t *score_class, double *scores,
+                         uint32_t *counts, hipStream_t stream) {
+    const uint64_t n = num_people * num_scores;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, part, cpart,
+                       splits, num_people, num_scores, limbs, nclass, score_class, scores, counts);
+    return hipGetLastError();
+}
+
+__global__ void k_fill_f64(double *p, uint64_t n, double v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+hipError_t launch_fill_f64(double *p, uint64_t n, double value, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = (n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536;
+    hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)blocks), dim3(256), 0, stream, p, n, value);
+    return hipGetLastError();
+}
+hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t value, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = (n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, stream, p, n, value);
+    return hipGetLastError();
+}
+
+}  // namespace gnm

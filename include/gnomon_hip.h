@@ -1,0 +1,192 @@
+/*
+ * gnomon_hip.h -- C ABI of the MI355X (gfx950) polygenic-scoring backend.
+ *
+ * This library is a drop-in for what sits behind gnomon's GPU entry point
+ *   score/cuda_backend.rs:604  try_run_cuda(&PipelineContext)
+ *       -> Result<Option<(Vec<f64>, Vec<u32>)>, PipelineError>
+ * called from score/pipeline.rs:480.  Every entry point below names the
+ * reference code it replaces.  No C++ exception, abort or allocation crosses
+ * the ABI; all pointers are plain host or device addresses and sizes.
+ *
+ * Semantics (identical to the reference):
+ *   scores[p*K + k] = baseline[k] + sum_v W_eff[v,k]*d_pv - sum_{v: missing} corr[v,k]
+ *                   = sum over CSR entries (v,k) with d_pv not missing of corr + W_eff*d
+ *   counts[p*K + k] = number of CSR entries (v,k) whose genotype for p is missing
+ * with d the A2-allele dosage decoded from the SNP-major PLINK .bed row
+ * (sample j: byte j>>2, bits 2(j&3); 00->0, 10->1, 11->2, 01->missing).
+ * Output layout is [person][score], persons in output order
+ * (output_idx_to_fam_idx), as score/cuda_backend.rs:1674-1679, 2063.
+ */
+#ifndef GNOMON_HIP_H
+#define GNOMON_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GNM_ABI_VERSION 1
+
+/* Replaces PipelineError (shared/pipeline_error.rs:5-10) plus the Ok(None)
+ * fallback of try_run_cuda (cuda_backend.rs:608-634). */
+typedef enum gnm_status {
+    GNM_OK = 0,
+    GNM_DECLINED = 1,          /* below the device size gate: caller runs the CPU path */
+    GNM_ERR_NO_DEVICE = 2,     /* no usable gfx950 device, or devices hidden by env */
+    GNM_ERR_INVALID_ARG = 3,
+    GNM_ERR_OUT_OF_MEMORY = 4,
+    GNM_ERR_HIP = 5,           /* a HIP runtime call failed after start */
+    GNM_ERR_INTERNAL = 6,
+    GNM_ERR_UNSUPPORTED = 7
+} gnm_status;
+
+/* Arithmetic of the genotype x weight contraction.
+ *   FP32:   plain FP32 wavefront reduction (FP32 weights, exact products, FP32
+ *           accumulation over <=256-variant blocks, FP64 across blocks), the
+ *           arithmetic class of the reference kernels (score/kernel.rs:124-176,
+ *           cuda_backend.rs:2554-2584).
+ *   BF16X2/BF16X3: split-BF16 MFMA.  Each weight column is written as 2 or 3
+ *           integer-valued BF16 limbs times a power-of-two scale per
+ *           (variant block, column); dosages 0/1/2 are exact in BF16, so every
+ *           product and every FP32 partial sum inside a variant block is exact;
+ *           blocks are combined in FP64.  Replaces the split-FP16 tensor-core
+ *           path (cuda_backend.rs:2360-2415).
+ *   AUTO:   BF16X3. */
+typedef enum gnm_precision {
+    GNM_PRECISION_FP32 = 0,
+    GNM_PRECISION_BF16X2 = 1,
+    GNM_PRECISION_BF16X3 = 2,
+    GNM_PRECISION_AUTO = 3
+} gnm_precision;
+
+/* The slice of PreparationResult (score/types.rs:166-365) the hot path reads. */
+typedef struct gnm_problem {
+    uint64_t num_people;            /* num_people_to_score (N) */
+    uint64_t total_people_in_fam;   /* .fam rows (N_fam) */
+    uint64_t bytes_per_variant;     /* ceil(N_fam / 4) (prepare.rs:811) */
+    uint64_t num_scores;            /* score_names.len() (K) */
+    uint64_t num_variants;          /* num_reconciled_variants (V, CSR rows) */
+    uint64_t nnz;                   /* sparse_row_offsets[V] */
+    const float *sparse_weights;              /* [nnz] W_eff, A2-dosage space */
+    const float *sparse_missing_corrections;  /* [nnz] corr */
+    const uint32_t *sparse_score_columns;     /* [nnz] column < K */
+    const uint64_t *sparse_row_offsets;       /* [V+1] */
+    const uint32_t *output_idx_to_fam_idx;    /* [N]; NULL = identity */
+    int32_t precision;              /* gnm_precision */
+    int32_t device;                 /* HIP ordinal; <0 = 0 */
+    uint64_t reserved[4];           /* must be zero */
+} gnm_problem;
+
+typedef struct gnm_ctx gnm_ctx;
+
+/* Thread-local description of the last failure on this thread. */
+const char *gnm_last_error(void);
+int gnm_abi_version(void);
+
+/* Size gate of try_run_cuda (cuda_backend.rs:38-39, 608-617): 1 when
+ * N*K >= 1e5 or N*K*V >= 1e8, else 0. */
+int gnm_size_gate(uint64_t num_people, uint64_t num_scores, uint64_t num_variants);
+
+/* Device probe, replaces cuda_driver_likely_available (cuda_backend.rs:675-695):
+ * 0 when HIP_VISIBLE_DEVICES / ROCR_VISIBLE_DEVICES is "", "-1" or "none",
+ * or when no gfx950 device is present; else the number of devices. */
+int gnm_device_count(void);
+
+/* Replaces CudaRuntime::new (cuda_backend.rs:994-1235): validates the CSR,
+ * builds the device weight plan (flip-aware effect-allele rows, split limbs,
+ * membership classes), allocates accumulators.  GNM_DECLINED when below the
+ * size gate (no context is created). */
+gnm_status gnm_create(const gnm_problem *problem, gnm_ctx **out_ctx);
+/* Same, skipping the size gate (tests and explicit GPU requests). */
+gnm_status gnm_create_forced(const gnm_problem *problem, gnm_ctx **out_ctx);
+
+/* Replaces the mega-batch body of process_dense_stream_cuda /
+ * run_pending_compute_cuda (cuda_backend.rs:1827-1978, 2067-2333).
+ * rows: num_rows packed .bed rows (bytes_per_variant each, row stride given)
+ * for reconciled variants [first_variant, first_variant + num_rows), in
+ * ascending order, on the host.  The call copies the rows through pinned
+ * staging and returns once they have been consumed (the caller may reuse
+ * the buffer); compute stays asynchronous. */
+gnm_status gnm_score_rows(gnm_ctx *ctx, const uint8_t *rows, uint64_t row_stride,
+                          uint64_t first_variant, uint64_t num_rows);
+/* Same with rows already resident in device memory (device-resident .bed). */
+gnm_status gnm_score_rows_device(gnm_ctx *ctx, const uint8_t *d_rows, uint64_t row_stride,
+                                 uint64_t first_variant, uint64_t num_rows);
+
+/* Checkpoint contract (score/checkpoint.rs:293-339, cuda_backend.rs:1969-2011):
+ * exact f64 sums (baseline included, as the reference stores them) and u32
+ * counts for variants [0, completed_variants). */
+gnm_status gnm_read_accumulators(gnm_ctx *ctx, double *scores, uint32_t *counts,
+                                 uint64_t *completed_variants);
+/* Resume (pipeline.rs:1777-1799, cuda_backend.rs:1680-1694): seed the
+ * accumulators from a checkpoint taken after completed_variants variants. */
+gnm_status gnm_seed_accumulators(gnm_ctx *ctx, const double *scores, const uint32_t *counts,
+                                 uint64_t completed_variants);
+
+/* Final result: waits for all work and copies N*K f64 scores (baseline
+ * included) and u32 missing counts to host buffers owned by the caller. */
+gnm_status gnm_finish(gnm_ctx *ctx, double *scores, uint32_t *counts);
+
+/* Synchronises and frees every device resource in a fixed order
+ * (cuda_backend.rs:2014-2058).  NULL is accepted. */
+void gnm_destroy(gnm_ctx *ctx);
+
+/* One-shot form of try_run_cuda over a host .bed image: rows for all V
+ * reconciled variants (row_stride apart).  GNM_DECLINED = caller falls back. */
+gnm_status gnm_run(const gnm_problem *problem, const uint8_t *rows, uint64_t row_stride,
+                   double *scores, uint32_t *counts);
+
+/* ---- device-pointer entry points (PyTorch-ROCm autograd path, tests) ------
+ * All pointers are device addresses; stream is a hipStream_t (NULL = default).
+ * The .bed block is num_variants rows of bytes_per_variant, persons
+ * [0, num_people) of each row (identity person map). */
+
+/* Standalone unpack: dosage[v*num_people + p] = reference tile value
+ * (score/batch.rs:489-503): effect-allele dosage 0/1/2 (2-d when flip[v]),
+ * 3 = missing; missing_per_person[p] += number of missing calls in the block
+ * (accumulated, caller zeroes).  flip may be NULL.  Replaces unpack_plink
+ * (cuda_backend.rs:70-125) as a separate stage. */
+gnm_status gnm_unpack_device(const uint8_t *d_rows, uint64_t bytes_per_variant,
+                             uint64_t num_variants, uint64_t num_people, const uint8_t *d_flip,
+                             uint8_t *d_dosage, uint32_t *d_missing_per_person, void *stream);
+
+/* Dense effect-allele forward, S[p,k] = sum_v g_pv W[v,k] with
+ * g = d (flip 0) or 2-d (flip 1), missing -> 0; missing_per_person[p] =
+ * number of missing calls.  W is [num_variants][num_scores] f32, S is
+ * [num_people][num_scores] f64 (overwritten).  New (no reference counterpart):
+ * the forward of the differentiable path. */
+gnm_status gnm_dense_forward_device(const uint8_t *d_rows, uint64_t bytes_per_variant,
+                                    uint64_t num_variants, uint64_t num_people,
+                                    const uint8_t *d_flip, const float *d_weights,
+                                    uint64_t num_scores, int32_t precision, double *d_scores,
+                                    uint32_t *d_missing_per_person, void *stream);
+
+/* Adjoint dL/dW = G^T dL/dS, recomputing G from the .bed (no stored
+ * dosages).  grad_scores [num_people][num_scores] f32, grad_weights
+ * [num_variants][num_scores] f32 (overwritten).  New (no reference counterpart). */
+gnm_status gnm_dense_adjoint_device(const uint8_t *d_rows, uint64_t bytes_per_variant,
+                                    uint64_t num_variants, uint64_t num_people,
+                                    const uint8_t *d_flip, const float *d_grad_scores,
+                                    uint64_t num_scores, int32_t precision,
+                                    float *d_grad_weights, void *stream);
+
+/* Synthetic SNP-major rows on the device, bit-identical to the test oracle's
+ * generator (variant v, sample s keyed counter-based hash; HWE with A2
+ * frequency in [p_lo24, p_hi24] * 2^-24; missing with probability
+ * miss16 * 2^-16).  Benchmark/test utility, not a reference function. */
+gnm_status gnm_synth_rows_device(uint64_t seed, uint32_t p_lo24, uint32_t p_hi24,
+                                 uint32_t miss16, uint64_t num_samples, uint64_t first_variant,
+                                 uint64_t num_rows, uint8_t *d_out, uint64_t row_stride,
+                                 void *stream);
+
+/* Kernel timing of the last gnm_score_rows[_device] call on this context,
+ * measured with HIP events on the library's compute stream: milliseconds of
+ * the main contraction kernel, and of the whole call (all kernels). */
+gnm_status gnm_last_timing(gnm_ctx *ctx, float *main_kernel_ms, float *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,107 @@
+"""C ABI checks that need no GPU: the library loads, exports every entry
+point declared in include/gnomon_hip.h, and rejects bad input before it
+touches a device."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from gnomon_amd import _lib
+from gnomon_amd.prep import csr_from_dense
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "gnomon_hip.h")).read()
+    return sorted(set(re.findall(r"\b(gnm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert _declared() == sorted(_lib.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    for name in _declared():
+        assert hasattr(L, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (gnm_\w+)", out))
+    assert set(_declared()) <= exported
+    assert L.gnm_abi_version() == 1
+
+
+def test_size_gate_matches_reference():
+    L = _lib.lib()
+    # cuda_backend.rs:38-39, 608-617: N*K >= 1e5 or N*K*V >= 1e8
+    assert L.gnm_size_gate(2000, 1, 50000) == 1          # 1e8 exactly
+    assert L.gnm_size_gate(2000, 1, 49999) == 0
+    assert L.gnm_size_gate(100000, 1, 1) == 1
+    assert L.gnm_size_gate(99999, 1, 1000) == 0
+    assert L.gnm_size_gate(5, 3, 10) == 0
+
+
+def test_devices_hidden_by_env(monkeypatch):
+    L = _lib.lib()
+    for val in ("", "-1", "none", " NONE "):
+        monkeypatch.setenv("HIP_VISIBLE_DEVICES", val)
+        assert L.gnm_device_count() == 0
+
+
+def _problem(csr, N, **kw):
+    pr = _lib.Problem()
+    pr.num_people = N
+    pr.total_people_in_fam = kw.get("nfam", N)
+    pr.bytes_per_variant = kw.get("bpv", (pr.total_people_in_fam + 3) // 4)
+    pr.num_scores = csr.num_scores
+    pr.num_variants = csr.num_variants
+    pr.nnz = csr.nnz
+    pr.sparse_weights = csr.weights.ctypes.data
+    pr.sparse_missing_corrections = csr.corrections.ctypes.data
+    pr.sparse_score_columns = csr.columns.ctypes.data
+    pr.sparse_row_offsets = csr.row_offsets.ctypes.data
+    pr.precision = kw.get("precision", 2)
+    return pr
+
+
+def test_invalid_arguments_rejected_without_device():
+    L = _lib.lib()
+    csr = csr_from_dense(np.ones((4, 2), np.float32), np.zeros(4, bool))
+    ctx = ctypes.c_void_p()
+    assert L.gnm_create_forced(None, ctypes.byref(ctx)) == _lib.GNM_ERR_INVALID_ARG
+    pr = _problem(csr, 10, bpv=2)  # wrong bytes_per_variant
+    assert L.gnm_create_forced(ctypes.byref(pr), ctypes.byref(ctx)) == _lib.GNM_ERR_INVALID_ARG
+    assert "bytes_per_variant" in _lib.last_error()
+    bad = csr_from_dense(np.ones((4, 2), np.float32), np.zeros(4, bool))
+    bad.columns[1] = 7
+    pr = _problem(bad, 10)
+    assert L.gnm_create_forced(ctypes.byref(pr), ctypes.byref(ctx)) == _lib.GNM_ERR_INVALID_ARG
+    pr = _problem(csr, 10, precision=9)
+    assert L.gnm_create_forced(ctypes.byref(pr), ctypes.byref(ctx)) == _lib.GNM_ERR_INVALID_ARG
+    pr = _problem(csr, 10, nfam=12)  # subset without a person map
+    assert L.gnm_create_forced(ctypes.byref(pr), ctypes.byref(ctx)) == _lib.GNM_ERR_INVALID_ARG
+    assert not ctx.value
+
+
+def test_small_problem_declines_and_no_device_is_reported(monkeypatch):
+    L = _lib.lib()
+    csr = csr_from_dense(np.ones((4, 2), np.float32), np.zeros(4, bool))
+    ctx = ctypes.c_void_p()
+    pr = _problem(csr, 10)
+    assert L.gnm_create(ctypes.byref(pr), ctypes.byref(ctx)) == _lib.GNM_DECLINED
+    monkeypatch.setenv("HIP_VISIBLE_DEVICES", "-1")
+    assert L.gnm_create_forced(ctypes.byref(pr), ctypes.byref(ctx)) == _lib.GNM_ERR_NO_DEVICE
+    from gnomon_amd import try_run_hip
+    assert try_run_hip(csr, np.zeros((4, 3), np.uint8), 10) is None
+    L.gnm_destroy(None)
+
+
+def test_oracle_is_not_imported_by_the_product():
+    for dirpath, _, files in os.walk(os.path.join(ROOT, "gnomon_amd")):
+        for f in files:
+            if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")):
+                text = open(os.path.join(dirpath, f), errors="replace").read()
+                assert "oracle" not in re.sub(r"#.*|//.*", "", text).replace("the oracle", ""), f

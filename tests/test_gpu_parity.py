@@ -1,0 +1,286 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+Integer work (decoded dosages, missing counts) must be bit-exact.  Scores:
+  * fp32   -- the reference's own GPU/CPU rule (score/tests/gpu_tests.rs:134-137):
+              |a-b| <= 1e-6 or |a-b|/max(|a|,|b|,1) <= 1e-6, against FP64;
+  * bf16x3 -- the same rule, and normwise max|S-S64|/max|S64| <= 1e-6 per score;
+  * bf16x2 -- normwise <= 1e-4 (16-bit limbs; reported, not a production mode).
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from gnomon_amd import _lib
+from gnomon_amd.pipeline import ScoringContext, try_run_hip
+from gnomon_amd.prep import ScoreCsr, csr_from_dense
+from tests.helpers import gpu_tests_genotype, normwise_rel, pack_bed, reference_agree
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))
+LD = os.path.join(os.path.dirname(__file__), "golden", "ld.bed")
+TOL = {"fp32": None, "bf16x3": 1e-6, "bf16x2": 1e-4}
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _check_scores(S, C, S64, C64, precision):
+    assert np.array_equal(C, C64), "missing counts differ"
+    if precision == "bf16x2":
+        assert normwise_rel(S, S64).max() <= TOL["bf16x2"]
+        return
+    ok = reference_agree(S, S64, 1e-6)
+    assert ok.all(), f"{(~ok).sum()} cells outside the 1e-6 rule; worst {np.abs(S - S64).max()}"
+    if precision == "bf16x3":
+        assert normwise_rel(S, S64).max() <= TOL["bf16x3"]
+
+
+def _score(csr, rows, N, precision, nfam=None, o2f=None, blocks=1):
+    with ScoringContext(csr, N, nfam, o2f, precision=precision, force=True) as ctx:
+        V = csr.num_variants
+        edges = np.linspace(0, V, blocks + 1).astype(int)
+        for a, b in zip(edges[:-1], edges[1:]):
+            ctx.score_rows(rows[a:b], int(a))
+        return ctx.finish()
+
+
+def test_device_visible():
+    assert _lib.lib().gnm_device_count() >= 1
+
+
+def test_synth_device_matches_cpu_generator(orc):
+    _flush("synth N=1001 V=300")
+    N, V, stride = 1001, 300, 260
+    d = torch.zeros(V * stride, dtype=torch.uint8, device="cuda")
+    _lib.check(_lib.lib().gnm_synth_rows_device(99, 838861, 8388608, 1311, N, 5, V,
+                                                ctypes.c_void_p(d.data_ptr()), stride, None))
+    torch.cuda.synchronize()
+    got = d.cpu().numpy().reshape(V, stride)[:, :(N + 3) // 4]
+    want = orc.synth_rows(99, N, 5, V, 838861, 8388608, 1311)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("N", [64, 1001, 4099])
+def test_unpack_bit_exact(orc, N):
+    _flush(f"unpack N={N}")
+    rng = np.random.default_rng(N)
+    V = 517
+    dos = rng.choice([0, 1, 2, -1], size=(V, N), p=[0.45, 0.3, 0.2, 0.05])
+    dos[3] = -1          # all missing
+    dos[4] = 2           # all hom A2
+    rows = pack_bed(dos)
+    flip = (rng.random(V) < 0.5).astype(np.uint8)
+    d_rows, d_flip = _dev(rows), _dev(flip)
+    d_out = torch.zeros(V * N, dtype=torch.uint8, device="cuda")
+    d_miss = torch.zeros(N, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    _lib.check(_lib.lib().gnm_unpack_device(ctypes.c_void_p(d_rows.data_ptr()), rows.shape[1], V, N,
+                                            ctypes.c_void_p(d_flip.data_ptr()),
+                                            ctypes.c_void_p(d_out.data_ptr()),
+                                            ctypes.c_void_p(d_miss.data_ptr()), None))
+    torch.cuda.synchronize()
+    tiles = orc.decode_rows(rows, N)
+    flipped = np.array([2, 1, 0, 3], np.uint8)[tiles]
+    want = np.where(flip[:, None] == 1, flipped, tiles)
+    assert np.array_equal(d_out.cpu().numpy().reshape(V, N), want)
+    assert np.array_equal(d_miss.cpu().numpy().astype(np.int64), (tiles == 3).sum(0))
+
+
+def test_unpack_ld_fixture(orc):
+    raw = np.fromfile(LD, np.uint8)
+    N, V = 200, 1500
+    rows = raw[3:].reshape(V, N // 4)
+    d_rows = _dev(rows)
+    d_out = torch.zeros(V * N, dtype=torch.uint8, device="cuda")
+    d_miss = torch.zeros(N, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    _lib.check(_lib.lib().gnm_unpack_device(ctypes.c_void_p(d_rows.data_ptr()), N // 4, V, N, None,
+                                            ctypes.c_void_p(d_out.data_ptr()),
+                                            ctypes.c_void_p(d_miss.data_ptr()), None))
+    torch.cuda.synchronize()
+    assert np.array_equal(d_out.cpu().numpy().reshape(V, N), orc.decode_rows(rows, N))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16x2"])
+def test_plain_small_config(orc, precision):
+    """2,000 people x 50,000 variants x 1 score, HWE p~U(0.05,0.5), 1% missing,
+    w ~ N(0, 0.01^2), effect allele A1 for a random half."""
+    _flush(f"plain_small {precision}")
+    N, V = 2000, 50_000
+    rows = orc.synth_rows(1, N, 0, V, int(0.05 * 2**24), int(0.5 * 2**24), 655)
+    rng = np.random.default_rng(2)
+    csr = csr_from_dense(rng.normal(0, 0.01, (V, 1)).astype(np.float32), rng.random(V) < 0.5)
+    S64, C64 = orc.score_fp64(rows, csr, N)
+    S, C = _score(csr, rows, N, precision, blocks=3)
+    _check_scores(S, C, S64, C64, precision)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_many_scores_classes_and_mixed_flips(orc, precision):
+    """K=70 with per-cell presence (membership classes > 4) and per-cell flips
+    (variants needing both an A2 and an A1 row); N not a multiple of 4."""
+    _flush(f"classes {precision}")
+    rng = np.random.default_rng(7)
+    N, V, K = 1234, 3000, 70
+    dos = rng.choice([0, 1, 2, -1], size=(V, N), p=[0.5, 0.3, 0.17, 0.03])
+    rows = pack_bed(dos)
+    W = rng.laplace(0, 1e-3, (V, K)).astype(np.float32)
+    W[rng.random((V, K)) < 0.3] = 0.0
+    flip = rng.random((V, K)) < 0.5
+    present = rng.random((V, K)) < 0.85
+    present[:, :3] = True   # a few identical classes
+    csr = csr_from_dense(W, flip, present)
+    S64, C64 = orc.score_fp64(rows, csr, N)
+    S, C = _score(csr, rows, N, precision, blocks=2)
+    _check_scores(S, C, S64, C64, precision)
+    # the restated reference engine agrees with both under the reference rule
+    Sr, Cr, _ = orc.refcpu_run(rows, csr, N, threads=4)
+    assert np.array_equal(Cr, C)
+    assert reference_agree(S, Sr, 2e-6).all()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_person_subset_gather(orc, precision):
+    _flush(f"subset {precision}")
+    rng = np.random.default_rng(9)
+    Nfam, V, K = 1001, 2500, 4
+    dos = rng.choice([0, 1, 2, -1], size=(V, Nfam), p=[0.4, 0.35, 0.2, 0.05])
+    rows = pack_bed(dos)
+    o2f = rng.permutation(Nfam)[:600].astype(np.uint32)
+    csr = csr_from_dense(rng.normal(0, 0.01, (V, K)).astype(np.float32), rng.random(V) < 0.5)
+    S64, C64 = orc.score_fp64(rows, csr, len(o2f), o2f)
+    S, C = _score(csr, rows, len(o2f), precision, nfam=Nfam, o2f=o2f)
+    _check_scores(S, C, S64, C64, precision)
+
+
+def _single(w_eff, corr, K=1):
+    return ScoreCsr(weights=np.full(K, w_eff, np.float32), corrections=np.full(K, corr, np.float32),
+                    columns=np.arange(K, dtype=np.uint32), row_offsets=np.array([0, K], np.uint64),
+                    num_scores=K, baseline=np.full(K, corr, np.float64),
+                    score_variant_counts=np.ones(K, np.uint32))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_reference_golden_vectors_on_device(precision):
+    _flush(f"golden {precision}")
+    g = GOLD["variant_major_missing_correction"]
+    S, C = _score(_single(g["w_eff"], g["corr"]), np.array([[g["byte"]]], np.uint8), 4, precision)
+    assert np.allclose(S[:, 0] - g["corr"], g["partial_scores"], atol=1e-12)
+    assert C[:, 0].tolist() == g["missing"]
+    g = GOLD["word_skip_and_tail"]
+    row = np.zeros((1, 26), np.uint8)
+    for person, code in g["codes"]:
+        row[0, person // 4] |= code << ((person % 4) * 2)
+    S, C = _score(_single(g["w_eff"], g["corr"]), row, 101, precision)
+    partial = S[:, 0] - g["corr"]
+    for p, val in g["partial_scores"].items():
+        assert partial[int(p)] == val
+    assert np.count_nonzero(partial) == g["nonzero_scores"] and C.sum() == 1 and C[99, 0] == 1
+    g = GOLD["many_scores"]
+    S, C = _score(_single(1.0, 0.0, g["num_scores"]), pack_bed(np.array([g["dosages"]])), 3, precision)
+    for p, expect in enumerate(g["scores"]):
+        assert np.all(S[p] == expect)
+    g = GOLD["special_case"]
+    csr = csr_from_dense(np.full((50, 1), g["weight"], np.float32), np.ones(50, bool))
+    S, C = _score(csr, pack_bed(np.array(g["a2_dosages"])[:, None]), 1, precision)
+    assert abs(S[0, 0] - g["sum"]) < 1e-6 and C[0, 0] == 0
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_split_self_test_known_answer(precision):
+    """Inputs of the reference's split self-test (cuda_backend.rs:2432-2438):
+    dosage[p][v] = (64p+v) % 3, weight[v][k] = ((7(64v+k)) mod 101 - 50)/13.
+    The exact answer is an integer sum divided by 13."""
+    n = 64
+    i = np.arange(n * n)
+    dos = (i % 3).reshape(n, n)                     # [people][variants]
+    num = ((i * 7) % 101 - 50).reshape(n, n)        # [variants][scores], weight = num / 13
+    W = (num / 13.0).astype(np.float32)
+    csr = csr_from_dense(W, np.zeros(n, bool))
+    S, C = _score(csr, pack_bed(dos.T), n, precision)
+    exact = (dos.astype(np.int64) @ num.astype(np.int64)) / 13.0
+    W64 = W.astype(np.float64)
+    exact_f32w = dos.astype(np.float64) @ W64
+    assert np.all(C == 0)
+    assert reference_agree(S, exact_f32w, 1e-6).all()
+    assert np.abs(S - exact).max() < 1e-4   # f32 weights vs the rational ninths
+
+
+def test_streaming_checkpoint_contract(orc):
+    """After a block, read_accumulators returns what the reference checkpoints
+    (baseline of all variants + contributions of the completed ones); a new
+    context seeded from it finishes with the full result."""
+    _flush("checkpoint")
+    rng = np.random.default_rng(4)
+    N, V, K = 777, 4000, 3
+    dos = rng.choice([0, 1, 2, -1], size=(V, N), p=[0.4, 0.35, 0.2, 0.05])
+    rows = pack_bed(dos)
+    csr = csr_from_dense(rng.normal(0, 0.01, (V, K)).astype(np.float32), rng.random(V) < 0.5)
+    S64, C64 = orc.score_fp64(rows, csr, N)
+    cut = 1500
+    with ScoringContext(csr, N, precision="bf16x3", force=True) as ctx:
+        ctx.score_rows(rows[:cut], 0)
+        Sck, Cck, done = ctx.read_accumulators()
+    assert done == cut
+    head = ScoreCsr(weights=csr.weights[:int(csr.row_offsets[cut])],
+                    corrections=csr.corrections[:int(csr.row_offsets[cut])],
+                    columns=csr.columns[:int(csr.row_offsets[cut])],
+                    row_offsets=csr.row_offsets[:cut + 1], num_scores=K,
+                    baseline=csr.baseline, score_variant_counts=csr.score_variant_counts)
+    Sh, Ch = orc.score_fp64(rows[:cut], head, N)
+    tail_base = np.zeros(K)
+    np.add.at(tail_base, csr.columns[int(csr.row_offsets[cut]):].astype(int),
+              csr.corrections[int(csr.row_offsets[cut]):].astype(np.float64))
+    assert np.array_equal(Cck, Ch)
+    assert reference_agree(Sck, Sh + tail_base, 1e-6).all()
+    with ScoringContext(csr, N, precision="bf16x3", force=True) as ctx2:
+        ctx2.seed_accumulators(Sck, Cck, cut)
+        ctx2.score_rows(rows[cut:], cut)
+        S, C = ctx2.finish()
+    _check_scores(S, C, S64, C64, "bf16x3")
+
+
+def test_try_run_hip_drop_in(orc):
+    _flush("try_run_hip")
+    rng = np.random.default_rng(12)
+    N, V, K = 2000, 50_000, 1
+    rows = orc.synth_rows(3, N, 0, V, int(0.05 * 2**24), int(0.5 * 2**24), 655)
+    csr = csr_from_dense(rng.normal(0, 0.01, (V, K)).astype(np.float32), rng.random(V) < 0.5)
+    out = try_run_hip(csr, rows, N)
+    assert out is not None
+    S64, C64 = orc.score_fp64(rows, csr, N)
+    _check_scores(out[0], out[1], S64, C64, "bf16x3")
+    # below the gate the backend declines
+    assert try_run_hip(csr_from_dense(np.ones((10, 1), np.float32), np.zeros(10, bool)),
+                       rows[:10], N) is None
+
+
+def test_device_resident_rows_match_host_rows(orc):
+    _flush("device-resident")
+    rng = np.random.default_rng(21)
+    N, V, K = 3000, 20_000, 8
+    stride = 768
+    d = torch.zeros(V * stride, dtype=torch.uint8, device="cuda")
+    _lib.check(_lib.lib().gnm_synth_rows_device(5, 167772, 8388608, 1311, N, 0, V,
+                                                ctypes.c_void_p(d.data_ptr()), stride, None))
+    torch.cuda.synchronize()
+    rows = orc.synth_rows(5, N, 0, V, 167772, 8388608, 1311)
+    csr = csr_from_dense(rng.normal(0, 0.01, (V, K)).astype(np.float32), rng.random(V) < 0.5)
+    S64, C64 = orc.score_fp64(rows, csr, N)
+    for precision in ("fp32", "bf16x3"):
+        with ScoringContext(csr, N, precision=precision, force=True) as ctx:
+            ctx.score_rows_device(d.data_ptr(), stride, 0, V)
+            S, C = ctx.finish()
+            main_ms, total_ms = ctx.last_timing()
+            assert 0 < main_ms <= total_ms
+        _check_scores(S, C, S64, C64, precision)
