@@ -18,6 +18,7 @@
 // persistent device accumulators.
 #include "../../include/gnomon_hip.h"
 #include "gnm_internal.hpp"
+#include "gnm_dense.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -338,12 +339,13 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
 
     if (!mfma_weights) {
         const uint64_t k32 = (K + kFp32ScoreChunk - 1) / kFp32ScoreChunk * kFp32ScoreChunk;
-        std::vector<float> w32(Rpad * k32, 0.0f);
+        // FP32 weights (rounded once from the effect-allele row value), held as f64
+        std::vector<double> w64(Rpad * k32, 0.0);
         for (uint64_t r = 0; r < R; ++r)
-            for (uint64_t k = 0; k < K; ++k) w32[r * k32 + k] = (float)wrow[r * K + k];
-        float *d_w32;
-        if ((st = c->upload(&d_w32, w32)) != GNM_OK) return st;
-        pl.w32 = d_w32;
+            for (uint64_t k = 0; k < K; ++k) w64[r * k32 + k] = (double)(float)wrow[r * K + k];
+        double *d_w64;
+        if ((st = c->upload(&d_w64, w64)) != GNM_OK) return st;
+        pl.w64 = d_w64;
         pl.k32 = k32;
     }
     const uint32_t L = c->limbs;
@@ -777,14 +779,185 @@ gnm_status gnm_synth_rows_device(uint64_t seed, uint32_t p_lo24, uint32_t p_hi24
     }
 }
 
-gnm_status gnm_dense_forward_device(const uint8_t *, uint64_t, uint64_t, uint64_t, const uint8_t *,
-                                    const float *, uint64_t, int32_t, double *, uint32_t *, void *) {
-    return fail(GNM_ERR_UNSUPPORTED, "gnm_dense_forward_device: not built yet");
+}  // extern "C"
+
+namespace {
+
+// Grow-only device scratch of the dense entry points, one set per device.
+// Calls on one device are expected to be serialised by the caller (they run
+// on the caller's stream); growing waits for that stream before freeing.
+struct DenseWorkspace {
+    std::vector<std::pair<void *, size_t>> bufs;
+    std::mutex mu;
+};
+std::mutex g_ws_mu;
+std::map<int, DenseWorkspace *> g_ws;
+
+gnm_status ws_get(DenseWorkspace *ws, size_t slot, size_t bytes, hipStream_t stream, void **out) {
+    if (ws->bufs.size() <= slot) ws->bufs.resize(slot + 1, {nullptr, 0});
+    auto &b = ws->bufs[slot];
+    bytes = std::max<size_t>(bytes, 256);
+    if (b.second < bytes) {
+        if (b.first) {
+            GNM_HIP(hipStreamSynchronize(stream));
+            GNM_HIP(hipFree(b.first));
+            b = {nullptr, 0};
+        }
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess)
+            return fail(e == hipErrorOutOfMemory ? GNM_ERR_OUT_OF_MEMORY : GNM_ERR_HIP,
+                        "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+        b = {p, bytes};
+    }
+    *out = b.first;
+    return GNM_OK;
 }
 
-gnm_status gnm_dense_adjoint_device(const uint8_t *, uint64_t, uint64_t, uint64_t, const uint8_t *,
-                                    const float *, uint64_t, int32_t, float *, void *) {
-    return fail(GNM_ERR_UNSUPPORTED, "gnm_dense_adjoint_device: not built yet");
+DenseWorkspace *workspace_for_current_device() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    auto it = g_ws.find(dev);
+    if (it == g_ws.end()) it = g_ws.emplace(dev, new DenseWorkspace()).first;
+    return it->second;
+}
+
+template <class T>
+gnm_status ws_typed(DenseWorkspace *ws, size_t slot, size_t count, hipStream_t st, T **out) {
+    void *p = nullptr;
+    gnm_status s = ws_get(ws, slot, count * sizeof(T), st, &p);
+    *out = (T *)p;
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+gnm_status gnm_dense_forward_device(const uint8_t *d_rows, uint64_t bytes_per_variant, uint64_t num_variants,
+                                    uint64_t num_people, const uint8_t *d_flip, const float *d_weights,
+                                    uint64_t num_scores, int32_t precision, double *d_scores,
+                                    uint32_t *d_missing_per_person, void *stream_) {
+    try {
+        const uint64_t V = num_variants, N = num_people, K = num_scores;
+        if (N == 0 || K == 0) return fail(GNM_ERR_INVALID_ARG, "num_people and num_scores must be positive");
+        if (V && (!d_rows || !d_weights)) return fail(GNM_ERR_INVALID_ARG, "NULL device pointer");
+        if (!d_scores || !d_missing_per_person) return fail(GNM_ERR_INVALID_ARG, "NULL output pointer");
+        if (bytes_per_variant < (N + 3) / 4) return fail(GNM_ERR_INVALID_ARG, "bytes_per_variant too small");
+        if (precision < GNM_PRECISION_FP32 || precision > GNM_PRECISION_AUTO)
+            return fail(GNM_ERR_INVALID_ARG, "unknown precision");
+        if (precision == GNM_PRECISION_AUTO) precision = GNM_PRECISION_BF16X3;
+        hipStream_t st = (hipStream_t)stream_;
+        DenseWorkspace *ws = workspace_for_current_device();
+        if (!ws) return fail(GNM_ERR_NO_DEVICE, "no current HIP device");
+        std::lock_guard<std::mutex> lk(ws->mu);
+        const uint32_t L = precision == GNM_PRECISION_BF16X2 ? 2 : (precision == GNM_PRECISION_BF16X3 ? 3 : 1);
+        const uint64_t Rpad = std::max<uint64_t>((V + kKStep - 1) / kKStep * kKStep, kKStep);
+        const uint32_t S = 8;
+        gnm_status s;
+        DevicePlan pl{};
+        uint32_t *prog_var, *flipmask, *prog_cls, *score_class, *cnt_tmp, *cpart;
+        double *part;
+        if ((s = ws_typed(ws, 0, Rpad, st, &prog_var)) != GNM_OK) return s;
+        if ((s = ws_typed(ws, 1, Rpad / 32, st, &flipmask)) != GNM_OK) return s;
+        if ((s = ws_typed(ws, 2, Rpad, st, &prog_cls)) != GNM_OK) return s;
+        if ((s = ws_typed(ws, 3, K, st, &score_class)) != GNM_OK) return s;
+        GNM_HIP(launch_fill_u32(score_class, K, 0u, st));
+        GNM_HIP(launch_dense_program(d_flip, V, Rpad, prog_var, flipmask, prog_cls, st));
+        pl.num_prog_rows = V;
+        pl.prog_var = prog_var;
+        pl.flipmask = flipmask;
+        pl.prog_cls = prog_cls;
+        pl.nclass = 1;
+        pl.score_class = score_class;
+        pl.limbs = L;
+        const uint64_t ncol_part = L == 1 ? K : (uint64_t)L * K;
+        if ((s = ws_typed(ws, 4, (uint64_t)S * N * ncol_part, st, &part)) != GNM_OK) return s;
+        if ((s = ws_typed(ws, 5, (uint64_t)S * N, st, &cpart)) != GNM_OK) return s;
+        if ((s = ws_typed(ws, 6, N * K, st, &cnt_tmp)) != GNM_OK) return s;
+        GNM_HIP(launch_fill_f64(d_scores, N * K, 0.0, st));
+        GNM_HIP(launch_fill_u32(cnt_tmp, N * K, 0u, st));
+        if (precision == GNM_PRECISION_FP32) {
+            const uint64_t k32 = (K + kFp32ScoreChunk - 1) / kFp32ScoreChunk * kFp32ScoreChunk;
+            double *w64;
+            if ((s = ws_typed(ws, 7, Rpad * k32, st, &w64)) != GNM_OK) return s;
+            GNM_HIP(launch_pad_weights(d_weights, V, K, Rpad, k32, w64, st));
+            pl.w64 = w64;
+            pl.k32 = k32;
+            const uint64_t rps = (V + S - 1) / S;
+            GNM_HIP(launch_score_fp32(pl, d_rows, bytes_per_variant, (N + 3) / 4, 0, 0, V, N, K, S, rps, true,
+                                      part, cpart, st));
+        } else {
+            pl.ncol = (uint32_t)(L * K);
+            pl.wtiles = (pl.ncol + 15) / 16;
+            pl.ctiles = 1;
+            const uint32_t tiles = pl.wtiles + pl.ctiles;
+            const uint64_t nks = Rpad / kKStep;
+            const uint64_t nblk = (Rpad + kFlushRows - 1) / kFlushRows;
+            int *exps;
+            uint16_t *frag;
+            double *scales;
+            if ((s = ws_typed(ws, 8, nblk * K, st, &exps)) != GNM_OK) return s;
+            if ((s = ws_typed(ws, 9, nks * tiles * 64 * 8, st, &frag)) != GNM_OK) return s;
+            if ((s = ws_typed(ws, 10, nblk * pl.ncol, st, &scales)) != GNM_OK) return s;
+            GNM_HIP(launch_block_exponent(d_weights, V, K, K, kFlushRows, L, exps, st));
+            GNM_HIP(launch_forward_frag(d_weights, V, K, L, exps, pl.wtiles, tiles, nks, frag, scales, st));
+            pl.bfrag = (const uint32_t *)frag;
+            pl.scales = scales;
+            uint64_t rps = (V + S - 1) / S;
+            rps = (rps + kFlushRows - 1) / kFlushRows * kFlushRows;
+            GNM_HIP(launch_score_mfma(pl, d_rows, bytes_per_variant, (N + 3) / 4, 0, 0, V, N, S, rps, 0, tiles,
+                                      part, cpart, st));
+        }
+        GNM_HIP(launch_reduce(part, cpart, S, N, K, L, 1, score_class, d_scores, cnt_tmp, st));
+        GNM_HIP(launch_take_col0(cnt_tmp, N, K, d_missing_per_person, st));
+        return GNM_OK;
+    } catch (...) {
+        return fail(GNM_ERR_INTERNAL, "exception in gnm_dense_forward_device");
+    }
+}
+
+gnm_status gnm_dense_adjoint_device(const uint8_t *d_rows, uint64_t bytes_per_variant, uint64_t num_variants,
+                                    uint64_t num_people, const uint8_t *d_flip, const float *d_grad_scores,
+                                    uint64_t num_scores, int32_t precision, float *d_grad_weights,
+                                    void *stream_) {
+    try {
+        const uint64_t V = num_variants, N = num_people, K = num_scores;
+        if (N == 0 || K == 0) return fail(GNM_ERR_INVALID_ARG, "num_people and num_scores must be positive");
+        if (V && (!d_rows || !d_grad_weights)) return fail(GNM_ERR_INVALID_ARG, "NULL device pointer");
+        if (!d_grad_scores) return fail(GNM_ERR_INVALID_ARG, "NULL grad_scores");
+        if (bytes_per_variant < (N + 3) / 4) return fail(GNM_ERR_INVALID_ARG, "bytes_per_variant too small");
+        if (precision == GNM_PRECISION_AUTO) precision = GNM_PRECISION_BF16X3;
+        if (precision != GNM_PRECISION_BF16X2 && precision != GNM_PRECISION_BF16X3)
+            return fail(GNM_ERR_UNSUPPORTED, "the adjoint runs in split-BF16 (bf16x2 or bf16x3)");
+        const uint32_t L = precision == GNM_PRECISION_BF16X2 ? 2 : 3;
+        const uint64_t Kp = (K + 15) / 16 * 16;
+        const uint32_t tiles = (uint32_t)(L * (Kp / 16));
+        if (tiles > 12) return fail(GNM_ERR_UNSUPPORTED, "adjoint supports up to 64 scores (bf16x3) per call");
+        hipStream_t st = (hipStream_t)stream_;
+        DenseWorkspace *ws = workspace_for_current_device();
+        if (!ws) return fail(GNM_ERR_NO_DEVICE, "no current HIP device");
+        std::lock_guard<std::mutex> lk(ws->mu);
+        const uint64_t nchunks = (N + 255) / 256;
+        const uint64_t nks = nchunks * 8;
+        const uint64_t nblk = (nchunks * 256 + kFlushRows - 1) / kFlushRows;
+        gnm_status s;
+        int *exps;
+        uint16_t *frag;
+        double *scales;
+        if ((s = ws_typed(ws, 11, nblk * K, st, &exps)) != GNM_OK) return s;
+        if ((s = ws_typed(ws, 12, nks * tiles * 64 * 8, st, &frag)) != GNM_OK) return s;
+        if ((s = ws_typed(ws, 13, nblk * L * Kp, st, &scales)) != GNM_OK) return s;
+        GNM_HIP(launch_fill_f64(scales, nblk * L * Kp, 0.0, st));
+        GNM_HIP(launch_block_exponent(d_grad_scores, N, K, K, kFlushRows, L, exps, st));
+        GNM_HIP(launch_adjoint_frag(d_grad_scores, N, K, Kp, L, exps, tiles, nks, frag, scales, st));
+        GNM_HIP(launch_adjoint(d_rows, bytes_per_variant, V, N, d_flip, (const uint32_t *)frag, scales, tiles, K, Kp,
+                               L, d_grad_weights, st));
+        return GNM_OK;
+    } catch (...) {
+        return fail(GNM_ERR_INTERNAL, "exception in gnm_dense_adjoint_device");
+    }
 }
 
 }  // extern "C"

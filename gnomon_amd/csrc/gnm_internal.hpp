@@ -15,12 +15,9 @@ constexpr int kKStep = 32;
 constexpr int kFlushRows = 8192;
 constexpr int kKStepsPerFlush = kFlushRows / kKStep;
 constexpr int kLimbBits = 8;
-// The FP32 path flushes its FP32 accumulators to FP64 every 256 variants, the
-// reference kernel's mini-batch (score/batch.rs:34).
-constexpr int kFp32FlushRows = 256;
 constexpr int kFp32ScoreChunk = 8;     // scores per pass of the FP32 kernel
 constexpr int kFp32MaxClasses = 4;     // membership classes counted in-kernel by the FP32 kernel
-constexpr int kMfmaMaxTiles = 12;      // 16-column tiles per pass of the MFMA kernel
+constexpr int kMfmaMaxTiles = 13;      // 16-column tiles per pass of the MFMA kernel (K=64 x 3 limbs + counts)
 constexpr int kPersonsPerMfmaBlock = 256;
 constexpr int kPersonsPerFp32Block = 1024;
 
@@ -31,7 +28,7 @@ struct DevicePlan {
     const uint32_t *flipmask;  // [ceil(R/32)] bit j of word s: row 32s+j reads the A1 allele
     const uint32_t *prog_cls;  // [R] membership-class bitmask of count rows (FP32 kernel, <=32 classes)
     // FP32 path
-    const float *w32;          // [R][k32] effect-allele weights (k32 = K rounded up to 8)
+    const double *w64;         // [R][k32] effect-allele FP32 weights held as f64 (k32 = K rounded up to 8)
     uint64_t k32;
     // split-BF16 path
     const uint32_t *bfrag;     // [R/32][tiles][64 lanes][4 dwords] B fragments (weights then counts)

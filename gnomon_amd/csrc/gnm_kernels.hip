@@ -176,10 +176,12 @@ hipError_t launch_compact(const uint8_t *rows, uint64_t stride, uint64_t num_row
 // ---------------------------------------------------------------------------
 // Plain FP32 path: a wavefront reduction with people on lanes (four per lane,
 // one packed byte) and the row's weights broadcast from scalar registers.
-// Per genotype and score: acc = fma(dosage, w, acc) -- bit-identical to the
-// reference kernel's add (dosage 1) and fma(w, 2, acc) (dosage 2)
-// (score/kernel.rs:34-44, 143-173).  FP32 accumulators are flushed to FP64
-// every 256 rows (score/batch.rs:34, 387-419).
+// FP32 weights, exact products (dosage 0/1/2 times an FP32 weight is exact in
+// FP64 as in FP32) and FP64 accumulation per genotype: v_fma_f64 runs at the
+// same rate as unpacked v_fma_f32 on CDNA4, and it removes the FP32 partial-sum
+// rounding of the reference kernel's 256-variant mini-batches
+// (score/kernel.rs:124-176, score/batch.rs:34, 387-419), which alone exceeds
+// 1e-6 relative at 1e5-1e6 variants.
 // ---------------------------------------------------------------------------
 template <int KC>
 __global__ __launch_bounds__(256) void k_score_fp32(DevicePlan plan, const uint8_t *__restrict__ rows,
@@ -198,56 +200,44 @@ __global__ __launch_bounds__(256) void k_score_fp32(DevicePlan plan, const uint8
     const uint64_t se = sb + rps;
     const uint64_t r_end = p1 < se ? p1 : se;
     const bool do_count = count_in_kernel && kc == 0;
-    const float *__restrict__ wbase = plan.w32 + (uint64_t)kc * KC;
+    const double *__restrict__ wbase = plan.w64 + (uint64_t)kc * KC;
 
-    double acc64[4][KC];
+    double acc[4][KC];
     uint32_t cnt[4][kFp32MaxClasses];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
 #pragma unroll
-        for (int k = 0; k < KC; ++k) acc64[t][k] = 0.0;
+        for (int k = 0; k < KC; ++k) acc[t][k] = 0.0;
 #pragma unroll
         for (int c = 0; c < kFp32MaxClasses; ++c) cnt[t][c] = 0;
     }
-    for (uint64_t r0 = r_begin; r0 < r_end; r0 += kFp32FlushRows) {
-        const uint64_t r1 = (r0 + kFp32FlushRows < r_end) ? r0 + kFp32FlushRows : r_end;
-        float acc[4][KC];
+    for (uint64_t r = r_begin; r < r_end; ++r) {
+        const uint64_t var = (uint64_t)(plan.prog_var[r] - (uint32_t)v0);
+        const uint32_t flip = (plan.flipmask[r >> 5] >> (r & 31)) & 1u;
+        uint32_t b = colok ? rows[var * stride + col] : 0x55u;
+        if (flip) b = flip_byte(b);
+        const double *__restrict__ w = wbase + r * plan.k32;
+        double wk[KC];
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int k = 0; k < KC; ++k) wk[k] = w[k];
 #pragma unroll
-            for (int k = 0; k < KC; ++k) acc[t][k] = 0.0f;
-        for (uint64_t r = r0; r < r1; ++r) {
-            const uint64_t var = (uint64_t)(plan.prog_var[r] - (uint32_t)v0);
-            const uint32_t flip = (plan.flipmask[r >> 5] >> (r & 31)) & 1u;
-            uint32_t b = colok ? rows[var * stride + col] : 0x55u;
-            if (flip) b = flip_byte(b);
-            const float *__restrict__ w = wbase + r * plan.k32;
-            float wk[KC];
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t c = (b >> (2 * t)) & 3u;
+            const double d = (double)((c >> 1) * ((c & 1u) + 1u));
 #pragma unroll
-            for (int k = 0; k < KC; ++k) wk[k] = w[k];
+            for (int k = 0; k < KC; ++k) acc[t][k] = __builtin_fma(d, wk[k], acc[t][k]);
+        }
+        if (do_count) {
+            const uint32_t cls = plan.prog_cls[r];
+            if (cls) {
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const uint32_t c = (b >> (2 * t)) & 3u;
-                const float d = (float)((c >> 1) * ((c & 1u) + 1u));
+                for (int t = 0; t < 4; ++t) {
+                    const uint32_t m = (((b >> (2 * t)) & 3u) == 1u) ? 1u : 0u;
 #pragma unroll
-                for (int k = 0; k < KC; ++k) acc[t][k] = __builtin_fmaf(d, wk[k], acc[t][k]);
-            }
-            if (do_count) {
-                const uint32_t cls = plan.prog_cls[r];
-                if (cls) {
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        const uint32_t m = (((b >> (2 * t)) & 3u) == 1u) ? 1u : 0u;
-#pragma unroll
-                        for (int c = 0; c < kFp32MaxClasses; ++c) cnt[t][c] += m & (cls >> c);
-                    }
+                    for (int c = 0; c < kFp32MaxClasses; ++c) cnt[t][c] += m & (cls >> c);
                 }
             }
         }
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-#pragma unroll
-            for (int k = 0; k < KC; ++k) acc64[t][k] += (double)acc[t][k];
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
@@ -256,7 +246,7 @@ __global__ __launch_bounds__(256) void k_score_fp32(DevicePlan plan, const uint8
         double *dst = part + ((uint64_t)s * num_people + p) * num_scores + (uint64_t)kc * KC;
 #pragma unroll
         for (int k = 0; k < KC; ++k)
-            if ((uint64_t)kc * KC + k < num_scores) dst[k] = acc64[t][k];
+            if ((uint64_t)kc * KC + k < num_scores) dst[k] = acc[t][k];
         if (do_count) {
             uint32_t *cd = cpart + ((uint64_t)s * num_people + p) * plan.nclass;
 #pragma unroll
@@ -310,15 +300,15 @@ union Frag {
     uint32_t u[4];
 };
 
-template <int TMAX>
-__global__ __launch_bounds__(256, 2) void k_score_mfma(DevicePlan plan, const uint8_t *__restrict__ rows,
-                                                       uint64_t stride, uint64_t bpv, uint64_t v0,
-                                                       uint64_t p0, uint64_t p1, uint64_t split_base,
-                                                       uint64_t rps, uint32_t splits,
-                                                       uint32_t person_tiles, uint64_t num_people,
-                                                       uint32_t tile_begin, uint32_t ntile,
-                                                       double *__restrict__ part,
-                                                       uint32_t *__restrict__ cpart) {
+template <int TMAX, int OCC>
+__global__ __launch_bounds__(256, OCC) void k_score_mfma(DevicePlan plan, const uint8_t *__restrict__ rows,
+                                                         uint64_t stride, uint64_t bpv, uint64_t v0,
+                                                         uint64_t p0, uint64_t p1, uint64_t split_base,
+                                                         uint64_t rps, uint32_t splits,
+                                                         uint32_t person_tiles, uint64_t num_people,
+                                                         uint32_t tile_begin, uint32_t ntile,
+                                                         double *__restrict__ part,
+                                                         uint32_t *__restrict__ cpart) {
     // XCD-aware placement: workgroups are dispatched round-robin over the 8
     // XCDs; every workgroup of one split lands on the same XCD so the split's
     // B fragments are shared through that XCD's L2.
@@ -341,6 +331,9 @@ __global__ __launch_bounds__(256, 2) void k_score_mfma(DevicePlan plan, const ui
     const uint32_t tiles_total = plan.wtiles + plan.ctiles;
     const uint32_t nw = tile_begin < wt ? ((wt - tile_begin) < ntile ? (wt - tile_begin) : ntile) : 0;
     const bool need_miss = nw < ntile;
+    const uint8_t *__restrict__ rowcol = rows + col;
+    // output row i = 4*lg + q of M-tile t is person pw0 + 16*lg + 4*q + t
+    __shared__ float stage[4][64 * 16];
 
     f32x4_t acc[4][TMAX];
 #pragma unroll
@@ -351,22 +344,25 @@ __global__ __launch_bounds__(256, 2) void k_score_mfma(DevicePlan plan, const ui
     bool first = true;
     uint64_t ks = r_begin / kKStep;
     const uint64_t ks_end = (r_end + kKStep - 1) / kKStep;
-    while (ks < ks_end) {
+    while (true) {
+        const bool empty = ks >= ks_end;
         const uint64_t blk = ks / kKStepsPerFlush;
         const uint64_t ks_stop = ((blk + 1) * kKStepsPerFlush < ks_end) ? (blk + 1) * kKStepsPerFlush : ks_end;
+#pragma unroll 1
         for (; ks < ks_stop; ++ks) {
-            const uint32_t fm = plan.flipmask[ks];
+            const uint32_t fm = plan.flipmask[ks] >> (lg * 8);
             uint32_t b[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const uint64_t r = ks * kKStep + lg * 8 + j;
-                const bool ok = (r >= r_begin) && (r < r_end) && colok;
-                uint32_t x = 0x55u;  // four missing calls: contributes nothing
-                if (ok) {
+                // rows outside this split (or people past the row) read as
+                // unflipped 00: dosage 0 and not missing, so they add nothing
+                uint32_t x = 0u;
+                if (r >= r_begin && r < r_end && colok) {
                     const uint64_t var = (uint64_t)(plan.prog_var[r] - (uint32_t)v0);
-                    x = rows[var * stride + col];
+                    x = rowcol[var * stride];
+                    if ((fm >> j) & 1u) x = flip_byte(x);
                 }
-                if ((fm >> (lg * 8 + j)) & 1u) x = flip_byte(x);
                 b[j] = x;
             }
             Frag A[4], M[4];
@@ -377,7 +373,7 @@ __global__ __launch_bounds__(256, 2) void k_score_mfma(DevicePlan plan, const ui
                 for (int t = 0; t < 4; ++t) {
                     const uint32_t x = ((pp >> (2 * t)) & 0x03030303u) | 0x04000400u;
                     A[t].u[q] = dose_pair(x);
-                    if (need_miss) M[t].u[q] = miss_pair(x);
+                    M[t].u[q] = need_miss ? miss_pair(x) : 0u;
                 }
             }
             const u32x4_t *bf = (const u32x4_t *)plan.bfrag + ((uint64_t)ks * tiles_total + tile_begin) * 64 + lane;
@@ -387,76 +383,57 @@ __global__ __launch_bounds__(256, 2) void k_score_mfma(DevicePlan plan, const ui
                     Frag B;
                     const u32x4_t bv = bf[(uint64_t)j * 64];
                     B.u[0] = bv[0]; B.u[1] = bv[1]; B.u[2] = bv[2]; B.u[3] = bv[3];
-                    if ((uint32_t)j < nw) {
+                    const bool w = (uint32_t)j < nw;
 #pragma unroll
-                        for (int t = 0; t < 4; ++t)
-                            acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[t].v, B.v, acc[t][j], 0, 0, 0);
-                    } else {
-#pragma unroll
-                        for (int t = 0; t < 4; ++t)
-                            acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(M[t].v, B.v, acc[t][j], 0, 0, 0);
-                    }
+                    for (int t = 0; t < 4; ++t)
+                        acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w ? A[t].v : M[t].v, B.v, acc[t][j], 0, 0, 0);
                 }
             }
         }
-        // flush this scale block: FP64 for weight columns, u32 for counts
+        // Flush this scale block: FP64 for weight columns, u32 for counts (an
+        // empty split writes zeros so its partial slots read zero).  Each tile
+        // is staged through LDS so the read-modify-write of the partials runs
+        // as one short loop with 128-byte rows instead of 64 scattered
+        // accesses held in registers.  All four waves of a workgroup cover the
+        // same rows, so they reach every flush together.
 #pragma unroll
         for (int j = 0; j < TMAX; ++j) {
             if ((uint32_t)j >= ntile) continue;
-            const uint32_t tile = tile_begin + j;
-            if (tile < wt) {
-                const uint32_t c = tile * 16 + li;
-                const double sc = (c < plan.ncol) ? plan.scales[blk * plan.ncol + c] : 0.0;
+            __syncthreads();
 #pragma unroll
-                for (int t = 0; t < 4; ++t)
+            for (int t = 0; t < 4; ++t) {
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const uint64_t p = pw0 + 16 * lg + 4 * q + t;
-                        if (p < num_people && c < plan.ncol) {
-                            const double v = (double)acc[t][j][q] * sc;
-                            double *dst = part + ((uint64_t)s * num_people + p) * plan.ncol + c;
-                            *dst = first ? v : *dst + v;
-                        }
-                    }
-            } else {
-                const uint32_t c = (tile - wt) * 16 + li;
-#pragma unroll
-                for (int t = 0; t < 4; ++t)
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const uint64_t p = pw0 + 16 * lg + 4 * q + t;
-                        if (p < num_people && c < plan.nclass) {
-                            const uint32_t v = (uint32_t)acc[t][j][q];
-                            uint32_t *dst = cpart + ((uint64_t)s * num_people + p) * plan.nclass + c;
-                            *dst = first ? v : *dst + v;
-                        }
-                    }
+                for (int q = 0; q < 4; ++q) stage[wave][(16 * lg + 4 * q + t) * 16 + li] = acc[t][j][q];
+                acc[t][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
             }
-#pragma unroll
-            for (int t = 0; t < 4; ++t) acc[t][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+            __syncthreads();
+            const uint32_t tile = tile_begin + j;
+            const bool wtile = tile < wt;
+            const uint32_t c = wtile ? tile * 16 + li : (tile - wt) * 16 + li;
+            const uint32_t ncols = wtile ? plan.ncol : plan.nclass;
+            if (c < ncols) {
+                const double sc = (wtile && !empty) ? plan.scales[blk * plan.ncol + c] : 0.0;
+#pragma unroll 1
+                for (int i = 0; i < 16; ++i) {
+                    const uint32_t pl = 16 * i / 4 + lg;   // 0..63, lanes of one lg share a person row
+                    const uint64_t p = pw0 + pl;
+                    if (p >= num_people) continue;
+                    const float v = stage[wave][pl * 16 + li];
+                    const uint64_t row = (uint64_t)s * num_people + p;
+                    if (wtile) {
+                        double *d = part + row * plan.ncol + c;
+                        const double x = (double)v * sc;
+                        *d = first ? x : *d + x;
+                    } else {
+                        uint32_t *d = cpart + row * plan.nclass + c;
+                        const uint32_t x = (uint32_t)v;
+                        *d = first ? x : *d + x;
+                    }
+                }
+            }
         }
         first = false;
-    }
-    if (first) {  // empty split: its partial slots still have to read zero
-#pragma unroll
-        for (int j = 0; j < TMAX; ++j) {
-            if ((uint32_t)j >= ntile) continue;
-            const uint32_t tile = tile_begin + j;
-#pragma unroll
-            for (int t = 0; t < 4; ++t)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const uint64_t p = pw0 + 16 * lg + 4 * q + t;
-                    if (p >= num_people) continue;
-                    if (tile < wt) {
-                        const uint32_t c = tile * 16 + li;
-                        if (c < plan.ncol) part[((uint64_t)s * num_people + p) * plan.ncol + c] = 0.0;
-                    } else {
-                        const uint32_t c = (tile - wt) * 16 + li;
-                        if (c < plan.nclass) cpart[((uint64_t)s * num_people + p) * plan.nclass + c] = 0u;
-                    }
-                }
-        }
+        if (ks >= ks_end) break;
     }
 }
 
@@ -471,11 +448,15 @@ hipError_t launch_score_mfma(const DevicePlan &plan, const uint8_t *rows, uint64
         const uint32_t nt = (tile_end - tb) < (uint32_t)kMfmaMaxTiles ? (tile_end - tb) : (uint32_t)kMfmaMaxTiles;
         const uint64_t wgs = (uint64_t)person_tiles * splits;  // splits is a multiple of 8
         if (nt <= 4) {
-            hipLaunchKernelGGL(k_score_mfma<4>, dim3((unsigned)wgs), dim3(256), 0, stream, plan, rows,
+            hipLaunchKernelGGL((k_score_mfma<4, 2>), dim3((unsigned)wgs), dim3(256), 0, stream, plan, rows,
+                               stride, bpv, v0, p0, p1, split_base, rows_per_split, splits,
+                               person_tiles, num_people, tb, nt, part, cpart);
+        } else if (nt <= 8) {
+            hipLaunchKernelGGL((k_score_mfma<8, 1>), dim3((unsigned)wgs), dim3(256), 0, stream, plan, rows,
                                stride, bpv, v0, p0, p1, split_base, rows_per_split, splits,
                                person_tiles, num_people, tb, nt, part, cpart);
         } else {
-            hipLaunchKernelGGL(k_score_mfma<kMfmaMaxTiles>, dim3((unsigned)wgs), dim3(256), 0, stream,
+            hipLaunchKernelGGL((k_score_mfma<kMfmaMaxTiles, 1>), dim3((unsigned)wgs), dim3(256), 0, stream,
                                plan, rows, stride, bpv, v0, p0, p1, split_base, rows_per_split,
                                splits, person_tiles, num_people, tb, nt, part, cpart);
         }

@@ -43,10 +43,11 @@ typedef enum gnm_status {
 } gnm_status;
 
 /* Arithmetic of the genotype x weight contraction.
- *   FP32:   plain FP32 wavefront reduction (FP32 weights, exact products, FP32
- *           accumulation over <=256-variant blocks, FP64 across blocks), the
- *           arithmetic class of the reference kernels (score/kernel.rs:124-176,
- *           cuda_backend.rs:2554-2584).
+ *   FP32:   plain FP32-weight wavefront reduction on the vector ALUs: FP32
+ *           weights, exact dosage x weight products, FP64 accumulation per
+ *           genotype (no split, no matrix cores).  Replaces the reference's
+ *           FP32 kernels (score/kernel.rs:124-176, cuda_backend.rs:2554-2584),
+ *           whose FP32 partial sums it makes exact.
  *   BF16X2/BF16X3: split-BF16 MFMA.  Each weight column is written as 2 or 3
  *           integer-valued BF16 limbs times a power-of-two scale per
  *           (variant block, column); dosages 0/1/2 are exact in BF16, so every
