@@ -10,7 +10,7 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgnomon_hip.so")
+LIB_PATH = os.environ.get("GNOMON_HIP_LIB") or os.path.join(HERE, "libgnomon_hip.so")
 
 GNM_OK = 0
 GNM_DECLINED = 1

@@ -114,6 +114,7 @@ struct gnm_ctx {
     uint8_t *h_stage[2] = {nullptr, nullptr};
     hipEvent_t stage_free[2] = {nullptr, nullptr};
     uint64_t stage_rows = 0;       // capacity of each staging slot in rows
+    uint64_t stage_pitch = 0;      // staging row pitch (bytes_per_variant rounded up to 64)
     hipStream_t stream = nullptr;
     hipEvent_t ev_main0 = nullptr, ev_main1 = nullptr, ev_all0 = nullptr, ev_all1 = nullptr;
     bool timing_valid = false;
@@ -270,8 +271,8 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
     }
     const uint64_t R = c->var_prog_off[V];
     c->R = R;
-    const uint64_t R32 = (R + kKStep - 1) / kKStep * kKStep;
-    const uint64_t Rpad = std::max<uint64_t>(R32, kKStep);
+    // padded to whole 64-row LDS stages of the MFMA kernel
+    const uint64_t Rpad = std::max<uint64_t>((R + 63) / 64 * 64, 64);
     std::vector<uint32_t> prog_var(Rpad, 0), flipmask(Rpad / 32, 0), prog_cls(Rpad, 0);
     std::vector<uint8_t> prog_count_row(Rpad, 0);
     for (uint64_t v = 0; v < V; ++v) {
@@ -438,14 +439,16 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
     if ((st = c->alloc(&c->d_counts, cells)) != GNM_OK) return st;
     GNM_HIP(launch_fill_f64(c->d_scores, cells, 0.0, c->stream));
     GNM_HIP(launch_fill_u32(c->d_counts, cells, 0u, c->stream));
-    c->splits = 8;
+    c->splits = mfma_weights ? mfma_splits(c->N, tiles) : fp32_splits(c->N, K);
+    if (!mfma_weights && !c->fp32_count_in_kernel)
+        c->splits = std::max(c->splits, mfma_splits(c->N, tiles));
     const uint64_t ncol_part = mfma_weights ? (uint64_t)L * K : K;
     if ((st = c->alloc(&c->d_part, (uint64_t)c->splits * c->N * ncol_part)) != GNM_OK) return st;
     if ((st = c->alloc(&c->d_cpart, (uint64_t)c->splits * c->N * c->nclass)) != GNM_OK) return st;
     if (!c->identity) {
         std::vector<uint32_t> o2f(p->output_idx_to_fam_idx, p->output_idx_to_fam_idx + c->N);
         if ((st = c->upload(&c->d_out_to_fam, o2f)) != GNM_OK) return st;
-        c->cbpv = (c->N + 3) / 4;
+        c->cbpv = ((c->N + 3) / 4 + 63) / 64 * 64;  // padded: 128-bit staging loads
     }
     GNM_HIP(hipEventCreate(&c->ev_main0));
     GNM_HIP(hipEventCreate(&c->ev_main1));
@@ -499,34 +502,28 @@ gnm_status score_device_rows(gnm_ctx *c, const uint8_t *d_rows, uint64_t stride,
                 c->d_compact = nullptr;
             }
             gnm_status st = c->alloc(&c->d_compact, n * c->cbpv);
+            if (st == GNM_OK) GNM_HIP(launch_fill_u32((uint32_t *)c->d_compact, n * c->cbpv / 4, 0u, c->stream));
             if (st != GNM_OK) return st;
             c->compact_rows = n;
         }
         GNM_HIP(launch_compact(d_rows, stride, n, c->d_out_to_fam, c->N, c->d_compact, c->cbpv, c->stream));
         rows = c->d_compact;
         rstride = c->cbpv;
-        bpv = c->cbpv;
+        bpv = (c->N + 3) / 4;
     }
     const uint64_t p0 = c->var_prog_off[v0], p1 = c->var_prog_off[v0 + n];
     const uint32_t S = c->splits;
+    const bool identity = (p1 - p0) == n;  // no mixed-flip variant in this block
     GNM_HIP(hipEventRecord(c->ev_main0, c->stream));
     if (c->precision == GNM_PRECISION_FP32) {
-        const uint64_t rps = (p1 - p0 + S - 1) / S;
-        GNM_HIP(launch_score_fp32(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, c->K, S, rps,
+        GNM_HIP(launch_score_fp32(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, c->K, S,
                                   c->fp32_count_in_kernel, c->d_part, c->d_cpart, c->stream));
-        if (!c->fp32_count_in_kernel) {
-            const uint64_t base = (p0 / kFlushRows) * kFlushRows;
-            uint64_t rps2 = (p1 - base + S - 1) / S;
-            rps2 = (rps2 + kFlushRows - 1) / kFlushRows * kFlushRows;
-            GNM_HIP(launch_score_mfma(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, S, rps2, 0,
-                                      c->plan.ctiles, c->d_part, c->d_cpart, c->stream));
-        }
+        if (!c->fp32_count_in_kernel)
+            GNM_HIP(launch_score_mfma(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, S, 0,
+                                      c->plan.ctiles, identity, c->d_part, c->d_cpart, c->stream));
     } else {
-        const uint64_t base = (p0 / kFlushRows) * kFlushRows;
-        uint64_t rps = (p1 - base + S - 1) / S;
-        rps = (rps + kFlushRows - 1) / kFlushRows * kFlushRows;
-        GNM_HIP(launch_score_mfma(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, S, rps, 0,
-                                  c->plan.wtiles + c->plan.ctiles, c->d_part, c->d_cpart, c->stream));
+        GNM_HIP(launch_score_mfma(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, S, 0,
+                                  c->plan.wtiles + c->plan.ctiles, identity, c->d_part, c->d_cpart, c->stream));
     }
     GNM_HIP(hipEventRecord(c->ev_main1, c->stream));
     GNM_HIP(launch_reduce(c->d_part, c->d_cpart, S, c->N, c->K,
@@ -632,10 +629,12 @@ gnm_status gnm_score_rows(gnm_ctx *ctx, const uint8_t *rows, uint64_t row_stride
         if (!c->h_stage[0]) {
             // two pinned slots of ~256 MiB (at least one row) -- the reference's
             // PIPELINE_SLOTS = 2 double buffer (cuda_backend.rs:42, 1137-1147)
-            c->stage_rows = std::max<uint64_t>(1, (256ull << 20) / bpv);
+            c->stage_pitch = (bpv + 63) / 64 * 64;  // 128-bit aligned rows for the staging loads
+            c->stage_rows = std::max<uint64_t>(1, (256ull << 20) / c->stage_pitch);
             for (int i = 0; i < 2; ++i) {
-                GNM_HIP(hipHostMalloc((void **)&c->h_stage[i], c->stage_rows * bpv, hipHostMallocDefault));
-                gnm_status s2 = c->alloc(&c->d_stage[i], c->stage_rows * bpv);
+                GNM_HIP(hipHostMalloc((void **)&c->h_stage[i], c->stage_rows * c->stage_pitch, hipHostMallocDefault));
+                std::memset(c->h_stage[i], 0, c->stage_rows * c->stage_pitch);
+                gnm_status s2 = c->alloc(&c->d_stage[i], c->stage_rows * c->stage_pitch);
                 if (s2 != GNM_OK) return s2;
                 GNM_HIP(hipEventCreateWithFlags(&c->stage_free[i], hipEventDisableTiming));
                 GNM_HIP(hipEventRecord(c->stage_free[i], c->stream));
@@ -647,13 +646,14 @@ gnm_status gnm_score_rows(gnm_ctx *ctx, const uint8_t *rows, uint64_t row_stride
             const uint64_t n = std::min(c->stage_rows, num_rows - done);
             GNM_HIP(hipEventSynchronize(c->stage_free[slot]));
             const uint8_t *src = rows + done * row_stride;
-            if (row_stride == bpv) {
-                std::memcpy(c->h_stage[slot], src, n * bpv);
+            const uint64_t pitch = c->stage_pitch;
+            if (row_stride == pitch) {
+                std::memcpy(c->h_stage[slot], src, n * pitch);
             } else {
-                for (uint64_t r = 0; r < n; ++r) std::memcpy(c->h_stage[slot] + r * bpv, src + r * row_stride, bpv);
+                for (uint64_t r = 0; r < n; ++r) std::memcpy(c->h_stage[slot] + r * pitch, src + r * row_stride, bpv);
             }
-            GNM_HIP(hipMemcpyAsync(c->d_stage[slot], c->h_stage[slot], n * bpv, hipMemcpyHostToDevice, c->stream));
-            st = score_device_rows(c, c->d_stage[slot], bpv, first_variant + done, n);
+            GNM_HIP(hipMemcpyAsync(c->d_stage[slot], c->h_stage[slot], n * pitch, hipMemcpyHostToDevice, c->stream));
+            st = score_device_rows(c, c->d_stage[slot], pitch, first_variant + done, n);
             if (st != GNM_OK) return st;
             GNM_HIP(hipEventRecord(c->stage_free[slot], c->stream));
             done += n;
@@ -853,8 +853,10 @@ gnm_status gnm_dense_forward_device(const uint8_t *d_rows, uint64_t bytes_per_va
         if (!ws) return fail(GNM_ERR_NO_DEVICE, "no current HIP device");
         std::lock_guard<std::mutex> lk(ws->mu);
         const uint32_t L = precision == GNM_PRECISION_BF16X2 ? 2 : (precision == GNM_PRECISION_BF16X3 ? 3 : 1);
-        const uint64_t Rpad = std::max<uint64_t>((V + kKStep - 1) / kKStep * kKStep, kKStep);
-        const uint32_t S = 8;
+        const uint64_t Rpad = std::max<uint64_t>((V + 63) / 64 * 64, 64);
+        const uint32_t Lw = L == 1 ? 0 : L;
+        const uint32_t S = precision == GNM_PRECISION_FP32 ? fp32_splits(N, K)
+                                                           : mfma_splits(N, (uint32_t)((Lw * K + 15) / 16 + 1));
         gnm_status s;
         DevicePlan pl{};
         uint32_t *prog_var, *flipmask, *prog_cls, *score_class, *cnt_tmp, *cpart;
@@ -885,8 +887,7 @@ gnm_status gnm_dense_forward_device(const uint8_t *d_rows, uint64_t bytes_per_va
             GNM_HIP(launch_pad_weights(d_weights, V, K, Rpad, k32, w64, st));
             pl.w64 = w64;
             pl.k32 = k32;
-            const uint64_t rps = (V + S - 1) / S;
-            GNM_HIP(launch_score_fp32(pl, d_rows, bytes_per_variant, (N + 3) / 4, 0, 0, V, N, K, S, rps, true,
+            GNM_HIP(launch_score_fp32(pl, d_rows, bytes_per_variant, (N + 3) / 4, 0, 0, V, N, K, S, true,
                                       part, cpart, st));
         } else {
             pl.ncol = (uint32_t)(L * K);
@@ -905,9 +906,7 @@ gnm_status gnm_dense_forward_device(const uint8_t *d_rows, uint64_t bytes_per_va
             GNM_HIP(launch_forward_frag(d_weights, V, K, L, exps, pl.wtiles, tiles, nks, frag, scales, st));
             pl.bfrag = (const uint32_t *)frag;
             pl.scales = scales;
-            uint64_t rps = (V + S - 1) / S;
-            rps = (rps + kFlushRows - 1) / kFlushRows * kFlushRows;
-            GNM_HIP(launch_score_mfma(pl, d_rows, bytes_per_variant, (N + 3) / 4, 0, 0, V, N, S, rps, 0, tiles,
+            GNM_HIP(launch_score_mfma(pl, d_rows, bytes_per_variant, (N + 3) / 4, 0, 0, V, N, S, 0, tiles, true,
                                       part, cpart, st));
         }
         GNM_HIP(launch_reduce(part, cpart, S, N, K, L, 1, score_class, d_scores, cnt_tmp, st));

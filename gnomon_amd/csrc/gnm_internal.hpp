@@ -17,9 +17,7 @@ constexpr int kKStepsPerFlush = kFlushRows / kKStep;
 constexpr int kLimbBits = 8;
 constexpr int kFp32ScoreChunk = 8;     // scores per pass of the FP32 kernel
 constexpr int kFp32MaxClasses = 4;     // membership classes counted in-kernel by the FP32 kernel
-constexpr int kMfmaMaxTiles = 13;      // 16-column tiles per pass of the MFMA kernel (K=64 x 3 limbs + counts)
-constexpr int kPersonsPerMfmaBlock = 256;
-constexpr int kPersonsPerFp32Block = 1024;
+constexpr int kMfmaMaxTiles = 8;       // 16-column tiles per pass of the MFMA kernel
 
 // Device-side view of the weight plan built by gnm_create.
 struct DevicePlan {
@@ -51,15 +49,20 @@ hipError_t launch_unpack(const uint8_t *rows, uint64_t stride, uint64_t num_rows
 hipError_t launch_compact(const uint8_t *rows, uint64_t stride, uint64_t num_rows,
                           const uint32_t *out_to_fam, uint64_t num_people, uint8_t *out,
                           uint64_t out_stride, hipStream_t stream);
+// Number of variant splits (a multiple of 8, 8..64) that fills the device.
+uint32_t mfma_splits(uint64_t num_people, uint32_t tiles);
+uint32_t fp32_splits(uint64_t num_people, uint64_t num_scores);
 hipError_t launch_score_fp32(const DevicePlan &plan, const uint8_t *rows, uint64_t stride,
                              uint64_t bpv, uint64_t v0, uint64_t p0, uint64_t p1,
                              uint64_t num_people, uint64_t num_scores, uint32_t splits,
-                             uint64_t rows_per_split, bool count_in_kernel, double *part,
-                             uint32_t *cpart, hipStream_t stream);
+                             bool count_in_kernel, double *part, uint32_t *cpart, hipStream_t stream);
+// identity: program rows [p0, p1) are the consecutive variants [v0, v0+p1-p0).
+// rows must be 16-byte aligned with a 16-byte multiple stride for the
+// 128-bit staging loads (else a byte-load variant runs)
 hipError_t launch_score_mfma(const DevicePlan &plan, const uint8_t *rows, uint64_t stride,
                              uint64_t bpv, uint64_t v0, uint64_t p0, uint64_t p1,
-                             uint64_t num_people, uint32_t splits, uint64_t rows_per_split,
-                             uint32_t tile_begin, uint32_t tile_end, double *part,
+                             uint64_t num_people, uint32_t splits,
+                             uint32_t tile_begin, uint32_t tile_end, bool identity, double *part,
                              uint32_t *cpart, hipStream_t stream);
 // scores[p][k] += sum_s sum_l part[s][p][l*K + k]; counts[p][k] += sum_s cpart[s][p][class[k]]
 hipError_t launch_reduce(const double *part, const uint32_t *cpart, uint32_t splits,

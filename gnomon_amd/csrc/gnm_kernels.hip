@@ -11,6 +11,8 @@
 // score/batch.rs:372-385) without a separate correction contraction.
 #include "gnm_internal.hpp"
 
+#include <type_traits>
+
 namespace gnm {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
@@ -211,30 +213,53 @@ __global__ __launch_bounds__(256) void k_score_fp32(DevicePlan plan, const uint8
 #pragma unroll
         for (int c = 0; c < kFp32MaxClasses; ++c) cnt[t][c] = 0;
     }
-    for (uint64_t r = r_begin; r < r_end; ++r) {
-        const uint64_t var = (uint64_t)(plan.prog_var[r] - (uint32_t)v0);
-        const uint32_t flip = (plan.flipmask[r >> 5] >> (r & 31)) & 1u;
-        uint32_t b = colok ? rows[var * stride + col] : 0x55u;
-        if (flip) b = flip_byte(b);
-        const double *__restrict__ w = wbase + r * plan.k32;
-        double wk[KC];
+    // eight rows per step, their loads issued together and one step ahead
+    constexpr int kRB = 8;
+    uint32_t nb[kRB];
+    auto load_rows = [&](uint64_t r0, uint32_t *out) {
 #pragma unroll
-        for (int k = 0; k < KC; ++k) wk[k] = w[k];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const uint32_t c = (b >> (2 * t)) & 3u;
-            const double d = (double)((c >> 1) * ((c & 1u) + 1u));
-#pragma unroll
-            for (int k = 0; k < KC; ++k) acc[t][k] = __builtin_fma(d, wk[k], acc[t][k]);
+        for (int i = 0; i < kRB; ++i) {
+            const uint64_t r = r0 + i;
+            uint32_t b = 0u;  // rows past the split: code 00 unflipped, weight rows unused
+            if (r < r_end) {
+                const uint64_t var = (uint64_t)(plan.prog_var[r] - (uint32_t)v0);
+                b = colok ? rows[var * stride + col] : 0x55u;
+                if ((plan.flipmask[r >> 5] >> (r & 31)) & 1u) b = flip_byte(b);
+            }
+            out[i] = b;
         }
-        if (do_count) {
-            const uint32_t cls = plan.prog_cls[r];
-            if (cls) {
+    };
+    if (r_begin < r_end) load_rows(r_begin, nb);
+    for (uint64_t r0 = r_begin; r0 < r_end; r0 += kRB) {
+        uint32_t cb[kRB];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const uint32_t m = (((b >> (2 * t)) & 3u) == 1u) ? 1u : 0u;
+        for (int i = 0; i < kRB; ++i) cb[i] = nb[i];
+        if (r0 + kRB < r_end) load_rows(r0 + kRB, nb);
 #pragma unroll
-                    for (int c = 0; c < kFp32MaxClasses; ++c) cnt[t][c] += m & (cls >> c);
+        for (int i = 0; i < kRB; ++i) {
+            const uint64_t r = r0 + i;
+            if (r >= r_end) break;
+            const uint32_t b = cb[i];
+            const double *__restrict__ w = wbase + r * plan.k32;
+            double wk[KC];
+#pragma unroll
+            for (int k = 0; k < KC; ++k) wk[k] = w[k];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const uint32_t c = (b >> (2 * t)) & 3u;
+                const double d = (double)((c >> 1) * ((c & 1u) + 1u));
+#pragma unroll
+                for (int k = 0; k < KC; ++k) acc[t][k] = __builtin_fma(d, wk[k], acc[t][k]);
+            }
+            if (do_count) {
+                const uint32_t cls = plan.prog_cls[r];
+                if (cls) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const uint32_t m = (((b >> (2 * t)) & 3u) == 1u) ? 1u : 0u;
+#pragma unroll
+                        for (int c = 0; c < kFp32MaxClasses; ++c) cnt[t][c] += m & (cls >> c);
+                    }
                 }
             }
         }
@@ -259,10 +284,10 @@ __global__ __launch_bounds__(256) void k_score_fp32(DevicePlan plan, const uint8
 hipError_t launch_score_fp32(const DevicePlan &plan, const uint8_t *rows, uint64_t stride,
                              uint64_t bpv, uint64_t v0, uint64_t p0, uint64_t p1,
                              uint64_t num_people, uint64_t num_scores, uint32_t splits,
-                             uint64_t rows_per_split, bool count_in_kernel, double *part,
-                             uint32_t *cpart, hipStream_t stream) {
+                             bool count_in_kernel, double *part, uint32_t *cpart, hipStream_t stream) {
     const uint64_t cols = (num_people + 3) / 4;
     const uint64_t split_base = p0;
+    const uint64_t rows_per_split = (p1 - p0 + splits - 1) / splits;
     dim3 grid((unsigned)((cols + 255) / 256), splits,
               (unsigned)((num_scores + kFp32ScoreChunk - 1) / kFp32ScoreChunk));
     hipLaunchKernelGGL(k_score_fp32<kFp32ScoreChunk>, grid, dim3(256), 0, stream, plan, rows,
@@ -295,20 +320,53 @@ __device__ __forceinline__ uint32_t miss_pair(uint32_t x) {
     return __builtin_amdgcn_perm(0x00003F00u, 0x00008000u, x);
 }
 
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS
+// operations and synchronises the waves, without the release fence of
+// __syncthreads() that would also drain every outstanding global load (the
+// prefetch ring must stay in flight across the barrier).
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 union Frag {
     bf16x8_t v;
     uint32_t u[4];
 };
 
-template <int TMAX, int OCC>
-__global__ __launch_bounds__(256, OCC) void k_score_mfma(DevicePlan plan, const uint8_t *__restrict__ rows,
-                                                         uint64_t stride, uint64_t bpv, uint64_t v0,
-                                                         uint64_t p0, uint64_t p1, uint64_t split_base,
-                                                         uint64_t rps, uint32_t splits,
-                                                         uint32_t person_tiles, uint64_t num_people,
-                                                         uint32_t tile_begin, uint32_t ntile,
-                                                         double *__restrict__ part,
-                                                         uint32_t *__restrict__ cpart) {
+__device__ __forceinline__ uint32_t flip_word(uint32_t b) {
+    return ~(((b & 0x55555555u) << 1) | ((b >> 1) & 0x55555555u));
+}
+
+// Workgroup of WAVES waves x 64 people = 16*WAVES packed bytes of every row.
+// Rows and B fragments move HBM/L2 -> registers -> LDS in stages of 64 rows
+// (two k-steps).  A ring of D stages is in flight per thread: at stage i the
+// loads of stage i+D-1 are issued, stage i (loaded D-1 stages earlier) is
+// written to one of two LDS buffers -- with the row's effect-allele flip
+// applied -- and, after one barrier, feeds the MFMAs.  Thread t loads the
+// 16-byte chunk (row t/WAVES, chunk t%WAVES) of every stage with one 128-bit
+// load; the LDS tile pitch is 16*WAVES+4 bytes so the eight rows a lane reads
+// sit on distinct banks.  Blocks whose program rows are consecutive variants
+// (no mixed-flip variant) skip the row-index indirection, so no load depends
+// on another load and the ring stays full.
+template <int TMAX, int WAVES, int D, bool VEC, bool IDENT>
+__global__ __launch_bounds__(64 * WAVES) void k_score_mfma(DevicePlan plan, const uint8_t *__restrict__ rows,
+                                                           uint64_t stride, uint64_t bpv, uint64_t v0,
+                                                           uint64_t p0, uint64_t p1, uint64_t split_base,
+                                                           uint64_t rps, uint32_t splits,
+                                                           uint32_t person_tiles, uint64_t num_people,
+                                                           uint32_t tile_begin, uint32_t ntile,
+                                                           double *__restrict__ part,
+                                                           uint32_t *__restrict__ cpart) {
+    constexpr int kBytes = 16 * WAVES;             // row bytes per workgroup
+    constexpr int kPitch = kBytes + 4;             // LDS row pitch
+    constexpr int kStage = 64;                     // rows per stage
+    constexpr int kNB = 2 * TMAX / WAVES;          // B chunks per thread per stage
+    static_assert(2 * TMAX % WAVES == 0, "B chunks must split evenly over the waves");
+    static_assert(D == 2 || D == 4, "pipeline depth");
+    __shared__ uint32_t tile[2][kStage * kPitch / 4];
+    __shared__ u32x4_t btile[2][2 * TMAX * 64];
+    __shared__ float stage[WAVES][64 * 16];
+
     // XCD-aware placement: workgroups are dispatched round-robin over the 8
     // XCDs; every workgroup of one split lands on the same XCD so the split's
     // B fragments are shared through that XCD's L2.
@@ -318,11 +376,11 @@ __global__ __launch_bounds__(256, OCC) void k_score_mfma(DevicePlan plan, const 
     const uint32_t ptile = slot % person_tiles;
     if (s >= splits) return;
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lg = lane >> 4;
-    const uint64_t pw0 = (uint64_t)ptile * kPersonsPerMfmaBlock + (uint64_t)wave * 64;
-    const uint64_t col = pw0 / 4 + li;
-    const bool colok = col < bpv;
+    const uint64_t col0 = (uint64_t)ptile * kBytes;
+    const uint64_t pw0 = (uint64_t)ptile * (64 * WAVES) + (uint64_t)wave * 64;
     const uint64_t sb = split_base + (uint64_t)s * rps;
     const uint64_t r_begin = p0 > sb ? p0 : sb;
     const uint64_t se = sb + rps;
@@ -331,9 +389,69 @@ __global__ __launch_bounds__(256, OCC) void k_score_mfma(DevicePlan plan, const 
     const uint32_t tiles_total = plan.wtiles + plan.ctiles;
     const uint32_t nw = tile_begin < wt ? ((wt - tile_begin) < ntile ? (wt - tile_begin) : ntile) : 0;
     const bool need_miss = nw < ntile;
-    const uint8_t *__restrict__ rowcol = rows + col;
-    // output row i = 4*lg + q of M-tile t is person pw0 + 16*lg + 4*q + t
-    __shared__ float stage[4][64 * 16];
+    const int ld_row = tid / WAVES, ld_chunk = tid % WAVES;
+    const uint64_t ld_col = col0 + (uint64_t)ld_chunk * 16;
+    const bool ld_colok = ld_col < bpv;
+    const u32x4_t *__restrict__ bfrag = (const u32x4_t *)plan.bfrag;
+
+    u32x4_t rb[D];          // row chunk of each stage in flight
+    u32x4_t rB[D][kNB];     // B chunks of each stage in flight
+    uint32_t rw[D];         // flip-mask word of this thread's row (raw load)
+    uint32_t rf[D];         // bits 0-4: flip bit index, bit 5: row inside the split
+
+    // Every load is issued unconditionally (clamped to a valid address, the
+    // value discarded at store time) so the compiler can count outstanding
+    // loads exactly and keep D-1 stages in flight across the barriers.
+    const uint64_t last_st = ((r_end - 1) / kStage) * kStage;
+    auto issue = [&](int d, uint64_t st_req) {
+        const uint64_t st = st_req < last_st ? st_req : last_st;
+        const uint64_t r = st + ld_row;
+        const bool ok = (st_req == st) && (r >= r_begin) && (r < r_end) && ld_colok;
+        const uint64_t rc = r < r_begin ? r_begin : (r >= r_end ? r_end - 1 : r);
+        const uint64_t var = IDENT ? (rc - p0) : (uint64_t)(plan.prog_var[rc] - (uint32_t)v0);
+        const uint64_t cc = ld_colok ? ld_col : 0;
+        const uint8_t *src = rows + var * stride + cc;
+        u32x4_t v;
+        if (VEC) {
+            v = *(const u32x4_t *)src;
+        } else {
+#pragma unroll
+            for (int dd = 0; dd < 4; ++dd) {
+                uint32_t x = 0;
+#pragma unroll
+                for (int bb = 0; bb < 4; ++bb) {
+                    const uint64_t o = cc + 4 * dd + bb;
+                    const uint32_t byte = rows[var * stride + (o < bpv ? o : bpv - 1)];
+                    x |= (o < bpv ? byte : 0u) << (8 * bb);
+                }
+                v[dd] = x;
+            }
+        }
+        rb[d] = v;
+        rw[d] = plan.flipmask[rc >> 5];  // used only at store time: no wait here
+        rf[d] = (uint32_t)(rc & 31) | (ok ? 32u : 0u);
+        const uint64_t ks0 = st / kKStep;
+#pragma unroll
+        for (int m = 0; m < kNB; ++m) {
+            const int q = tid + m * 64 * WAVES;       // chunk of the stage's B block
+            const int h = q / (TMAX * 64), j = (q / 64) % TMAX, l = q % 64;
+            const uint32_t jc = (uint32_t)j < ntile ? (uint32_t)j : ntile - 1;  // tiles >= ntile are never read
+            rB[d][m] = bfrag[((ks0 + h) * tiles_total + tile_begin + jc) * 64 + l];
+        }
+    };
+    auto store = [&](int buf, int d) {
+        u32x4_t v = rb[d];
+        const uint32_t f = rf[d];
+        if (!(f & 32u)) {
+            v = u32x4_t{0u, 0u, 0u, 0u};  // rows outside the split read as unflipped 00: add nothing
+        } else if ((rw[d] >> (f & 31u)) & 1u) {
+            v[0] = flip_word(v[0]); v[1] = flip_word(v[1]); v[2] = flip_word(v[2]); v[3] = flip_word(v[3]);
+        }
+        uint32_t *dst = &tile[buf][(ld_row * kPitch + ld_chunk * 16) / 4];
+        dst[0] = v[0]; dst[1] = v[1]; dst[2] = v[2]; dst[3] = v[3];
+#pragma unroll
+        for (int m = 0; m < kNB; ++m) btile[buf][tid + m * 64 * WAVES] = rB[d][m];
+    };
 
     f32x4_t acc[4][TMAX];
 #pragma unroll
@@ -341,30 +459,56 @@ __global__ __launch_bounds__(256, OCC) void k_score_mfma(DevicePlan plan, const 
 #pragma unroll
         for (int j = 0; j < TMAX; ++j) acc[t][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
 
-    bool first = true;
-    uint64_t ks = r_begin / kKStep;
-    const uint64_t ks_end = (r_end + kKStep - 1) / kKStep;
-    while (true) {
-        const bool empty = ks >= ks_end;
-        const uint64_t blk = ks / kKStepsPerFlush;
-        const uint64_t ks_stop = ((blk + 1) * kKStepsPerFlush < ks_end) ? (blk + 1) * kKStepsPerFlush : ks_end;
+    // Flush one scale block: FP64 for weight columns, u32 for counts.  Each
+    // tile is staged through LDS so the read-modify-write of the partials is
+    // one short loop over 128-byte rows.  `empty` writes zeros (a split
+    // without rows must still leave zeros in its partial slots).
+    auto flush = [&](uint64_t blk, bool first, bool empty) {
+#pragma unroll
+        for (int j = 0; j < TMAX; ++j) {
+            if ((uint32_t)j >= ntile) continue;
+            lds_barrier();
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) stage[wave][(16 * lg + 4 * q + t) * 16 + li] = acc[t][j][q];
+                acc[t][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+            }
+            lds_barrier();
+            const uint32_t tl = tile_begin + j;
+            const bool wtile = tl < wt;
+            const uint32_t c = wtile ? tl * 16 + li : (tl - wt) * 16 + li;
+            const uint32_t ncols = wtile ? plan.ncol : plan.nclass;
+            if (c < ncols) {
+                const double sc = (wtile && !empty) ? plan.scales[blk * plan.ncol + c] : 0.0;
 #pragma unroll 1
-        for (; ks < ks_stop; ++ks) {
-            const uint32_t fm = plan.flipmask[ks] >> (lg * 8);
+                for (int i = 0; i < 16; ++i) {
+                    const uint32_t pl = 4 * i + lg;  // person of the wave, 0..63
+                    const uint64_t p = pw0 + pl;
+                    if (p >= num_people) continue;
+                    const float v = stage[wave][pl * 16 + li];
+                    const uint64_t row = (uint64_t)s * num_people + p;
+                    if (wtile) {
+                        double *dd = part + row * plan.ncol + c;
+                        const double x = (double)v * sc;
+                        *dd = first ? x : *dd + x;
+                    } else {
+                        uint32_t *dd = cpart + row * plan.nclass + c;
+                        const uint32_t x = (uint32_t)v;
+                        *dd = first ? x : *dd + x;
+                    }
+                }
+            }
+        }
+    };
+
+    auto compute = [&](int buf, uint64_t st) {
+        const uint8_t *tb = (const uint8_t *)tile[buf];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
             uint32_t b[8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const uint64_t r = ks * kKStep + lg * 8 + j;
-                // rows outside this split (or people past the row) read as
-                // unflipped 00: dosage 0 and not missing, so they add nothing
-                uint32_t x = 0u;
-                if (r >= r_begin && r < r_end && colok) {
-                    const uint64_t var = (uint64_t)(plan.prog_var[r] - (uint32_t)v0);
-                    x = rowcol[var * stride];
-                    if ((fm >> j) & 1u) x = flip_byte(x);
-                }
-                b[j] = x;
-            }
+            for (int j = 0; j < 8; ++j) b[j] = tb[(32 * h + 8 * lg + j) * kPitch + wave * 16 + li];
             Frag A[4], M[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -376,12 +520,11 @@ __global__ __launch_bounds__(256, OCC) void k_score_mfma(DevicePlan plan, const 
                     M[t].u[q] = need_miss ? miss_pair(x) : 0u;
                 }
             }
-            const u32x4_t *bf = (const u32x4_t *)plan.bfrag + ((uint64_t)ks * tiles_total + tile_begin) * 64 + lane;
 #pragma unroll
             for (int j = 0; j < TMAX; ++j) {
                 if ((uint32_t)j < ntile) {
                     Frag B;
-                    const u32x4_t bv = bf[(uint64_t)j * 64];
+                    const u32x4_t bv = btile[buf][(h * TMAX + j) * 64 + lane];
                     B.u[0] = bv[0]; B.u[1] = bv[1]; B.u[2] = bv[2]; B.u[3] = bv[3];
                     const bool w = (uint32_t)j < nw;
 #pragma unroll
@@ -390,76 +533,101 @@ __global__ __launch_bounds__(256, OCC) void k_score_mfma(DevicePlan plan, const 
                 }
             }
         }
-        // Flush this scale block: FP64 for weight columns, u32 for counts (an
-        // empty split writes zeros so its partial slots read zero).  Each tile
-        // is staged through LDS so the read-modify-write of the partials runs
-        // as one short loop with 128-byte rows instead of 64 scattered
-        // accesses held in registers.  All four waves of a workgroup cover the
-        // same rows, so they reach every flush together.
-#pragma unroll
-        for (int j = 0; j < TMAX; ++j) {
-            if ((uint32_t)j >= ntile) continue;
-            __syncthreads();
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) stage[wave][(16 * lg + 4 * q + t) * 16 + li] = acc[t][j][q];
-                acc[t][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-            }
-            __syncthreads();
-            const uint32_t tile = tile_begin + j;
-            const bool wtile = tile < wt;
-            const uint32_t c = wtile ? tile * 16 + li : (tile - wt) * 16 + li;
-            const uint32_t ncols = wtile ? plan.ncol : plan.nclass;
-            if (c < ncols) {
-                const double sc = (wtile && !empty) ? plan.scales[blk * plan.ncol + c] : 0.0;
-#pragma unroll 1
-                for (int i = 0; i < 16; ++i) {
-                    const uint32_t pl = 16 * i / 4 + lg;   // 0..63, lanes of one lg share a person row
-                    const uint64_t p = pw0 + pl;
-                    if (p >= num_people) continue;
-                    const float v = stage[wave][pl * 16 + li];
-                    const uint64_t row = (uint64_t)s * num_people + p;
-                    if (wtile) {
-                        double *d = part + row * plan.ncol + c;
-                        const double x = (double)v * sc;
-                        *d = first ? x : *d + x;
-                    } else {
-                        uint32_t *d = cpart + row * plan.nclass + c;
-                        const uint32_t x = (uint32_t)v;
-                        *d = first ? x : *d + x;
-                    }
-                }
-            }
-        }
-        first = false;
-        if (ks >= ks_end) break;
+        (void)st;
+    };
+
+    if (r_begin >= r_end) {
+        flush(0, true, true);
+        return;
     }
+    const uint64_t st0 = (r_begin / kStage) * kStage;
+    const uint64_t nst = (r_end - st0 + kStage - 1) / kStage;
+#pragma unroll
+    for (int d = 0; d < D - 1; ++d) issue(d, st0 + (uint64_t)d * kStage);
+    bool first = true;
+    // One pipeline step per stage; `d` is a compile-time ring slot so the
+    // ring stays in fixed registers (no copies that would force waits).
+    auto step = [&](auto dc, uint64_t i) {
+        constexpr int d = decltype(dc)::value;
+        issue((d + D - 1) % D, st0 + (i + D - 1) * kStage);
+        const int buf = (int)(i & 1);
+        store(buf, d);
+        lds_barrier();
+        const uint64_t st = st0 + i * kStage;
+        compute(buf, st);
+        const uint64_t st_next = st + kStage;
+        if ((st_next % kFlushRows) == 0 || i + 1 == nst) {
+            flush(st / kFlushRows, first, false);
+            first = false;
+        }
+    };
+    for (uint64_t i0 = 0; i0 < nst; i0 += D) {
+        step(std::integral_constant<int, 0>{}, i0);
+        if (i0 + 1 >= nst) break;
+        step(std::integral_constant<int, 1>{}, i0 + 1);
+        if constexpr (D > 2) {
+            if (i0 + 2 >= nst) break;
+            step(std::integral_constant<int, 2>{}, i0 + 2);
+            if (i0 + 3 >= nst) break;
+            step(std::integral_constant<int, 3>{}, i0 + 3);
+        }
+    }
+}
+
+static uint32_t splits_for(uint64_t wgs_per_split) {
+    // enough workgroups to fill 256 CUs several times; a multiple of 8 so
+    // every XCD owns whole splits
+    uint64_t s = (3072 + wgs_per_split - 1) / (wgs_per_split ? wgs_per_split : 1);
+    s = (s + 7) / 8 * 8;
+    return (uint32_t)(s < 8 ? 8 : (s > 64 ? 64 : s));
+}
+
+static int mfma_waves(uint32_t ntile) { return ntile <= 4 ? 8 : 4; }
+
+uint32_t mfma_splits(uint64_t num_people, uint32_t tiles) {
+    const uint32_t nt = tiles < (uint32_t)kMfmaMaxTiles ? tiles : (uint32_t)kMfmaMaxTiles;
+    const uint64_t per = 64ull * mfma_waves(nt ? nt : 1);
+    return splits_for((num_people + per - 1) / per);
+}
+
+uint32_t fp32_splits(uint64_t num_people, uint64_t num_scores) {
+    const uint64_t pt = ((num_people + 3) / 4 + 255) / 256;
+    const uint64_t chunks = (num_scores + kFp32ScoreChunk - 1) / kFp32ScoreChunk;
+    return splits_for(pt * chunks);
 }
 
 hipError_t launch_score_mfma(const DevicePlan &plan, const uint8_t *rows, uint64_t stride,
                              uint64_t bpv, uint64_t v0, uint64_t p0, uint64_t p1,
-                             uint64_t num_people, uint32_t splits, uint64_t rows_per_split,
-                             uint32_t tile_begin, uint32_t tile_end, double *part,
+                             uint64_t num_people, uint32_t splits,
+                             uint32_t tile_begin, uint32_t tile_end, bool identity, double *part,
                              uint32_t *cpart, hipStream_t stream) {
-    const uint32_t person_tiles = (uint32_t)((num_people + kPersonsPerMfmaBlock - 1) / kPersonsPerMfmaBlock);
-    const uint64_t split_base = (p0 / kFlushRows) * kFlushRows;
+    const bool vec = ((reinterpret_cast<uintptr_t>(rows) | stride) & 15u) == 0;
+    // splits start on 64-row stage boundaries; scale blocks may straddle them
+    const uint64_t split_base = (p0 / 64) * 64;
+    uint64_t rows_per_split = (p1 - split_base + splits - 1) / splits;
+    rows_per_split = (rows_per_split + 63) / 64 * 64;
     for (uint32_t tb = tile_begin; tb < tile_end; tb += kMfmaMaxTiles) {
         const uint32_t nt = (tile_end - tb) < (uint32_t)kMfmaMaxTiles ? (tile_end - tb) : (uint32_t)kMfmaMaxTiles;
-        const uint64_t wgs = (uint64_t)person_tiles * splits;  // splits is a multiple of 8
-        if (nt <= 4) {
-            hipLaunchKernelGGL((k_score_mfma<4, 2>), dim3((unsigned)wgs), dim3(256), 0, stream, plan, rows,
-                               stride, bpv, v0, p0, p1, split_base, rows_per_split, splits,
-                               person_tiles, num_people, tb, nt, part, cpart);
-        } else if (nt <= 8) {
-            hipLaunchKernelGGL((k_score_mfma<8, 1>), dim3((unsigned)wgs), dim3(256), 0, stream, plan, rows,
-                               stride, bpv, v0, p0, p1, split_base, rows_per_split, splits,
-                               person_tiles, num_people, tb, nt, part, cpart);
-        } else {
-            hipLaunchKernelGGL((k_score_mfma<kMfmaMaxTiles, 1>), dim3((unsigned)wgs), dim3(256), 0, stream,
-                               plan, rows, stride, bpv, v0, p0, p1, split_base, rows_per_split,
-                               splits, person_tiles, num_people, tb, nt, part, cpart);
-        }
+#define GNM_LAUNCH_MFMA(TM, WV, DD, VC, ID)                                                                  \
+    do {                                                                                                     \
+        const uint32_t pt = (uint32_t)((num_people + 64 * WV - 1) / (64 * WV));                              \
+        const uint64_t wgs = (uint64_t)pt * splits; /* splits is a multiple of 8 */                          \
+        hipLaunchKernelGGL((k_score_mfma<TM, WV, DD, VC, ID>), dim3((unsigned)wgs), dim3(64 * WV), 0, stream,    \
+                           plan, rows, stride, bpv, v0, p0, p1, split_base, rows_per_split, splits, pt,      \
+                           num_people, tb, nt, part, cpart);                                                 \
+    } while (0)
+#define GNM_LAUNCH_MFMA_V(TM, WV, DD)                                                                        \
+    do {                                                                                                     \
+        if (vec) {                                                                                           \
+            if (identity) GNM_LAUNCH_MFMA(TM, WV, DD, true, true); else GNM_LAUNCH_MFMA(TM, WV, DD, true, false); \
+        } else {                                                                                             \
+            if (identity) GNM_LAUNCH_MFMA(TM, WV, DD, false, true); else GNM_LAUNCH_MFMA(TM, WV, DD, false, false); \
+        }                                                                                                    \
+    } while (0)
+        if (mfma_waves(nt) == 8) GNM_LAUNCH_MFMA_V(4, 8, 4);
+        else GNM_LAUNCH_MFMA_V(kMfmaMaxTiles, 4, 2);
+#undef GNM_LAUNCH_MFMA_V
+#undef GNM_LAUNCH_MFMA
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
