@@ -12,7 +12,7 @@ OUTPUT = os.path.join(HERE, "libgnomon_hip.so")
 # -packed-fp32-ops off: hipcc -O3 may otherwise form v_pk_{add,mul,fma}_f32 from
 # scalar float code; the host pass warns that it does not know the feature.
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]

@@ -12,6 +12,7 @@
 #include "gnm_internal.hpp"
 
 #include <type_traits>
+#include <utility>
 
 namespace gnm {
 
@@ -313,6 +314,15 @@ hipError_t launch_score_fp32(const DevicePlan &plan, const uint8_t *rows, uint64
 // Integer-valued limbs make each FP32 partial sum exact inside a scale block
 // of kFlushRows rows; blocks are scaled and combined in FP64.
 // ---------------------------------------------------------------------------
+// selector [c0, c0+4, c1, c1+4] from the two codes in bits 0-1 of bytes 0
+// and 2 (bytes 1 and 3 hold copies): (y & 0x03030303) | 0x04000400 as one
+// v_and_or_b32 (the compiler otherwise emits an and plus an or)
+__device__ __forceinline__ uint32_t sel_pair(uint32_t y) {
+    uint32_t x;
+    const uint32_t m = 0x03030303u, c = 0x04000400u;
+    asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(x) : "v"(y), "v"(m), "v"(c));
+    return x;
+}
 __device__ __forceinline__ uint32_t dose_pair(uint32_t x) {
     return __builtin_amdgcn_perm(0x403F0000u, 0x00800000u, x);
 }
@@ -362,7 +372,6 @@ __global__ __launch_bounds__(64 * WAVES) void k_score_mfma(DevicePlan plan, cons
     constexpr int kStage = 64;                     // rows per stage
     constexpr int kNB = 2 * TMAX / WAVES;          // B chunks per thread per stage
     static_assert(2 * TMAX % WAVES == 0, "B chunks must split evenly over the waves");
-    static_assert(D == 2 || D == 4, "pipeline depth");
     __shared__ uint32_t tile[2][kStage * kPitch / 4];
     __shared__ u32x4_t btile[2][2 * TMAX * 64];
     __shared__ float stage[WAVES][64 * 16];
@@ -402,8 +411,30 @@ __global__ __launch_bounds__(64 * WAVES) void k_score_mfma(DevicePlan plan, cons
     // Every load is issued unconditionally (clamped to a valid address, the
     // value discarded at store time) so the compiler can count outstanding
     // loads exactly and keep D-1 stages in flight across the barriers.
+    // Stages strictly inside [r_begin, r_end) of an identity block take the
+    // fast path: per-thread pointers advanced by uniform strides.
     const uint64_t last_st = ((r_end - 1) / kStage) * kStage;
-    auto issue = [&](int d, uint64_t st_req) {
+    // per-thread constants of the B-chunk addresses: chunk m is (h, j, l)
+    uint32_t boff[kNB];
+#pragma unroll
+    for (int m = 0; m < kNB; ++m) {
+        const int q = tid + m * 64 * WAVES;
+        const int h = q / (TMAX * 64), j = (q / 64) % TMAX, l = q % 64;
+        const uint32_t jc = (uint32_t)j < ntile ? (uint32_t)j : ntile - 1;  // tiles >= ntile are never read
+        boff[m] = (uint32_t)((h * tiles_total + tile_begin + jc) * 64 + l);
+    }
+    const uint8_t *bed_row0 = rows + (uint64_t)ld_row * stride + (ld_colok ? ld_col : 0);
+    const uint32_t rf_fast = (uint32_t)(ld_row & 31) | (ld_colok ? 32u : 0u);
+    auto issue = [&](int d, uint64_t st_req) __attribute__((always_inline)) {
+        if (IDENT && st_req >= r_begin && st_req + kStage <= r_end) {
+            rb[d] = *(const u32x4_t *)(bed_row0 + (st_req - p0) * stride);
+            rw[d] = plan.flipmask[(st_req >> 5) + (ld_row >> 5)];
+            rf[d] = rf_fast;
+            const u32x4_t *bs = bfrag + (st_req / kKStep) * tiles_total * 64;
+#pragma unroll
+            for (int m = 0; m < kNB; ++m) rB[d][m] = bs[boff[m]];
+            return;
+        }
         const uint64_t st = st_req < last_st ? st_req : last_st;
         const uint64_t r = st + ld_row;
         const bool ok = (st_req == st) && (r >= r_begin) && (r < r_end) && ld_colok;
@@ -430,22 +461,22 @@ __global__ __launch_bounds__(64 * WAVES) void k_score_mfma(DevicePlan plan, cons
         rb[d] = v;
         rw[d] = plan.flipmask[rc >> 5];  // used only at store time: no wait here
         rf[d] = (uint32_t)(rc & 31) | (ok ? 32u : 0u);
-        const uint64_t ks0 = st / kKStep;
+        const u32x4_t *bs = bfrag + (st / kKStep) * tiles_total * 64;
 #pragma unroll
-        for (int m = 0; m < kNB; ++m) {
-            const int q = tid + m * 64 * WAVES;       // chunk of the stage's B block
-            const int h = q / (TMAX * 64), j = (q / 64) % TMAX, l = q % 64;
-            const uint32_t jc = (uint32_t)j < ntile ? (uint32_t)j : ntile - 1;  // tiles >= ntile are never read
-            rB[d][m] = bfrag[((ks0 + h) * tiles_total + tile_begin + jc) * 64 + l];
-        }
+        for (int m = 0; m < kNB; ++m) rB[d][m] = bs[boff[m]];
     };
-    auto store = [&](int buf, int d) {
+    auto store = [&](int buf, int d) __attribute__((always_inline)) {
         u32x4_t v = rb[d];
         const uint32_t f = rf[d];
-        if (!(f & 32u)) {
-            v = u32x4_t{0u, 0u, 0u, 0u};  // rows outside the split read as unflipped 00: add nothing
-        } else if ((rw[d] >> (f & 31u)) & 1u) {
-            v[0] = flip_word(v[0]); v[1] = flip_word(v[1]); v[2] = flip_word(v[2]); v[3] = flip_word(v[3]);
+        // zero rows outside the split (they add nothing); flip A1 rows:
+        // code c -> c ^ u with u = 11 where both bits are equal (00 <-> 11)
+        const uint32_t keep = (f & 32u) ? 0xFFFFFFFFu : 0u;
+        const uint32_t fl = ((rw[d] >> (f & 31u)) & 1u) ? keep : 0u;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t x = v[e] & keep;
+            const uint32_t u = ~(x ^ (x >> 1)) & 0x55555555u;
+            v[e] = x ^ ((u | (u << 1)) & fl);
         }
         uint32_t *dst = &tile[buf][(ld_row * kPitch + ld_chunk * 16) / 4];
         dst[0] = v[0]; dst[1] = v[1]; dst[2] = v[2]; dst[3] = v[3];
@@ -463,10 +494,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_score_mfma(DevicePlan plan, cons
     // tile is staged through LDS so the read-modify-write of the partials is
     // one short loop over 128-byte rows.  `empty` writes zeros (a split
     // without rows must still leave zeros in its partial slots).
-    auto flush = [&](uint64_t blk, bool first, bool empty) {
-#pragma unroll
-        for (int j = 0; j < TMAX; ++j) {
-            if ((uint32_t)j >= ntile) continue;
+    auto flush = [&](uint64_t blk, bool first, bool empty) __attribute__((always_inline)) {
+        // compile-time tile index (fold below): acc[t][j] must stay in registers
+        auto flush_tile = [&](auto jc) __attribute__((always_inline)) {
+            constexpr int j = decltype(jc)::value;
+            if ((uint32_t)j >= ntile) return;
             lds_barrier();
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
@@ -499,77 +531,97 @@ __global__ __launch_bounds__(64 * WAVES) void k_score_mfma(DevicePlan plan, cons
                     }
                 }
             }
-        }
+        };
+        [&]<int... J>(std::integer_sequence<int, J...>) __attribute__((always_inline)) {
+            (flush_tile(std::integral_constant<int, J>{}), ...);
+        }(std::make_integer_sequence<int, TMAX>{});
     };
 
-    auto compute = [&](int buf, uint64_t st) {
+    // Per k-step: eight LDS bytes per lane -> selectors for the four M-tiles
+    // -> dose (and missing) fragments; every B fragment is read from LDS once
+    // and feeds the four M-tiles.
+    auto compute = [&](int buf) __attribute__((always_inline)) {
         const uint8_t *tb = (const uint8_t *)tile[buf];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            uint32_t b[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) b[j] = tb[(32 * h + 8 * lg + j) * kPitch + wave * 16 + li];
-            Frag A[4], M[4];
+            uint32_t x[4][4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const uint32_t pp = (b[2 * q] | (b[2 * q + 1] << 16)) * 0x0101u;
+                const uint32_t b0 = tb[(32 * h + 8 * lg + 2 * q) * kPitch + wave * 16 + li];
+                const uint32_t b1 = tb[(32 * h + 8 * lg + 2 * q + 1) * kPitch + wave * 16 + li];
+                const uint32_t pp = (b0 | (b1 << 16)) * 0x0101u;
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const uint32_t x = ((pp >> (2 * t)) & 0x03030303u) | 0x04000400u;
-                    A[t].u[q] = dose_pair(x);
-                    M[t].u[q] = need_miss ? miss_pair(x) : 0u;
-                }
+                for (int t = 0; t < 4; ++t) x[q][t] = sel_pair(pp >> (2 * t));
             }
+            Frag A[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) A[t].u[q] = dose_pair(x[q][t]);
 #pragma unroll
             for (int j = 0; j < TMAX; ++j) {
-                if ((uint32_t)j < ntile) {
+                if ((uint32_t)j < nw) {
                     Frag B;
                     const u32x4_t bv = btile[buf][(h * TMAX + j) * 64 + lane];
                     B.u[0] = bv[0]; B.u[1] = bv[1]; B.u[2] = bv[2]; B.u[3] = bv[3];
-                    const bool w = (uint32_t)j < nw;
 #pragma unroll
                     for (int t = 0; t < 4; ++t)
-                        acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w ? A[t].v : M[t].v, B.v, acc[t][j], 0, 0, 0);
+                        acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[t].v, B.v, acc[t][j], 0, 0, 0);
+                }
+            }
+            if (need_miss) {
+                Frag M[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) M[t].u[q] = miss_pair(x[q][t]);
+#pragma unroll
+                for (int j = 0; j < TMAX; ++j) {
+                    if ((uint32_t)j >= nw && (uint32_t)j < ntile) {
+                        Frag B;
+                        const u32x4_t bv = btile[buf][(h * TMAX + j) * 64 + lane];
+                        B.u[0] = bv[0]; B.u[1] = bv[1]; B.u[2] = bv[2]; B.u[3] = bv[3];
+#pragma unroll
+                        for (int t = 0; t < 4; ++t)
+                            acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(M[t].v, B.v, acc[t][j], 0, 0, 0);
+                    }
                 }
             }
         }
-        (void)st;
     };
 
     if (r_begin >= r_end) {
         flush(0, true, true);
         return;
     }
-    const uint64_t st0 = (r_begin / kStage) * kStage;
+    // Stages are processed in groups of D (one ring turn); the first stage is
+    // aligned down to a group boundary so scale-block boundaries (every
+    // kFlushRows rows, a multiple of D*kStage) always fall between groups and
+    // the flush stays out of the unrolled steps.
+    static_assert(kFlushRows % (D * kStage) == 0, "scale blocks must hold whole ring turns");
+    const uint64_t st0 = (r_begin / (D * kStage)) * (D * kStage);
     const uint64_t nst = (r_end - st0 + kStage - 1) / kStage;
 #pragma unroll
     for (int d = 0; d < D - 1; ++d) issue(d, st0 + (uint64_t)d * kStage);
     bool first = true;
     // One pipeline step per stage; `d` is a compile-time ring slot so the
     // ring stays in fixed registers (no copies that would force waits).
-    auto step = [&](auto dc, uint64_t i) {
+    auto step = [&](auto dc, uint64_t i) __attribute__((always_inline)) {
         constexpr int d = decltype(dc)::value;
         issue((d + D - 1) % D, st0 + (i + D - 1) * kStage);
         const int buf = (int)(i & 1);
         store(buf, d);
         lds_barrier();
-        const uint64_t st = st0 + i * kStage;
-        compute(buf, st);
-        const uint64_t st_next = st + kStage;
-        if ((st_next % kFlushRows) == 0 || i + 1 == nst) {
-            flush(st / kFlushRows, first, false);
-            first = false;
-        }
+        compute(buf);
     };
     for (uint64_t i0 = 0; i0 < nst; i0 += D) {
-        step(std::integral_constant<int, 0>{}, i0);
-        if (i0 + 1 >= nst) break;
-        step(std::integral_constant<int, 1>{}, i0 + 1);
-        if constexpr (D > 2) {
-            if (i0 + 2 >= nst) break;
-            step(std::integral_constant<int, 2>{}, i0 + 2);
-            if (i0 + 3 >= nst) break;
-            step(std::integral_constant<int, 3>{}, i0 + 3);
+        [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
+            ((i0 + I < nst ? (step(std::integral_constant<int, I>{}, i0 + I), 0) : 0), ...);
+        }(std::make_integer_sequence<int, D>{});
+        const uint64_t st_next = st0 + (i0 + D) * kStage;
+        if ((st_next % kFlushRows) == 0 || i0 + D >= nst) {
+            flush((st0 + i0 * kStage) / kFlushRows, first, false);
+            first = false;
         }
     }
 }
@@ -582,7 +634,7 @@ static uint32_t splits_for(uint64_t wgs_per_split) {
     return (uint32_t)(s < 8 ? 8 : (s > 64 ? 64 : s));
 }
 
-static int mfma_waves(uint32_t ntile) { return ntile <= 4 ? 8 : 4; }
+static int mfma_waves(uint32_t) { return 4; }
 
 uint32_t mfma_splits(uint64_t num_people, uint32_t tiles) {
     const uint32_t nt = tiles < (uint32_t)kMfmaMaxTiles ? tiles : (uint32_t)kMfmaMaxTiles;
@@ -624,7 +676,7 @@ hipError_t launch_score_mfma(const DevicePlan &plan, const uint8_t *rows, uint64
             if (identity) GNM_LAUNCH_MFMA(TM, WV, DD, false, true); else GNM_LAUNCH_MFMA(TM, WV, DD, false, false); \
         }                                                                                                    \
     } while (0)
-        if (mfma_waves(nt) == 8) GNM_LAUNCH_MFMA_V(4, 8, 4);
+        if (nt <= 4) GNM_LAUNCH_MFMA_V(4, 4, 4);
         else GNM_LAUNCH_MFMA_V(kMfmaMaxTiles, 4, 2);
 #undef GNM_LAUNCH_MFMA_V
 #undef GNM_LAUNCH_MFMA
