@@ -31,6 +31,64 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def calibrate_copy(mib):
+    """A plain elementwise kernel of known traffic: reads and writes mib MiB
+    (torch.bitwise_not on uint8), three launches."""
+    import torch
+
+    n = mib << 20
+    src = torch.randint(0, 256, (n,), dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.bitwise_not(src, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes_read": n, "calibrate_copy_bytes_written": n, "launches": 3}), flush=True)
+
+
+def measure_unpack(d_bed, N, V, bpv, stride, reps):
+    """Standalone unpack stage (gnm_unpack_device): packed rows -> one tile
+    byte per person and variant, effect-allele flip on half the rows, plus
+    per-person missing counts.  Algorithmic bytes = rows read + tiles written
+    + counts; timed with events on the stream the kernel is launched on."""
+    import ctypes
+
+    import torch
+
+    from gnomon_amd import _lib
+
+    free, _ = torch.cuda.mem_get_info()
+    vu = int(min(V, (free * 0.8) // N))
+    if vu <= 0:
+        return None
+    out = torch.empty(vu * N, dtype=torch.uint8, device="cuda")
+    miss = torch.zeros(N, dtype=torch.int32, device="cuda")
+    flip = (torch.arange(vu, device="cuda") % 2).to(torch.uint8)
+    stream = torch.cuda.current_stream()
+    L = _lib.lib()
+
+    def launch():
+        _lib.check(L.gnm_unpack_device(ctypes.c_void_p(d_bed.data_ptr()), stride, vu, N,
+                                       ctypes.c_void_p(flip.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                       ctypes.c_void_p(miss.data_ptr()), ctypes.c_void_p(stream.cuda_stream)))
+
+    launch()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        launch()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    alg = vu * bpv + vu * N + N * 8
+    gbs = alg / (ms / 1e3) / 1e9
+    log(f"unpack: {vu} variants, {ms:.3f} ms, {gbs:.1f} GB/s algorithmic")
+    del out
+    return {"kernel": "k_unpack", "variants": vu, "people": N, "ms": round(ms, 4), "algorithmic_bytes": alg,
+            "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -43,7 +101,14 @@ def main():
     ap.add_argument("--cpu-sample-variants", type=int, default=30000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="check a slice against the FP64 oracle")
+    ap.add_argument("--no-unpack", action="store_true", help="skip the standalone unpack-stage measurement")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain elementwise copy of MIB MiB (read + write) three times; used to scale "
+                         "the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}); the path's sharding is "
               "described in DESIGN.md and not measured here", file=sys.stderr)
@@ -133,6 +198,8 @@ def main():
                 "kernel": "k_score_mfma" if args.precision != "fp32" else "k_score_fp32",
                 "kernel_ms": round(main_ms, 4), "algorithmic_bytes": alg_bytes}
 
+    unpack = None if args.no_unpack else measure_unpack(d_bed, N, V, bpv, stride, max(3, min(args.steps, 10)))
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as orc
@@ -160,6 +227,7 @@ def main():
         "config": {"workload": w.name, "people": N, "variants": V, "scores": K,
                    "bed_bytes": V * bpv, "precision": args.precision},
         "roofline": roofline, "cpu_baseline": cpu, "compare": compare, "check": check,
+        "unpack_stage": unpack,
     }
     print(json.dumps(out), flush=True)
 

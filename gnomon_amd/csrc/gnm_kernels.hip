@@ -80,52 +80,234 @@ hipError_t launch_synth(uint64_t seed, uint32_t p_lo24, uint32_t p_hi24, uint32_
 }
 
 // ---------------------------------------------------------------------------
-// Standalone unpack: one thread per packed byte column (4 people) and a
-// chunk of rows; writes the reference tile values (0,1,2 effect dosage,
-// 3 missing) and accumulates per-person missing counts.
+// Standalone unpack: reference tile values (0,1,2 effect dosage, 3 missing)
+// [row][person] plus per-person missing counts.
+//
+// Tile values are formed on 16 packed fields at once: with l, h the low and
+// high bit of a code, value bit0 = l^h and bit1 = l (A2 dosage) or ~h (A1
+// dosage, flipped row); the 2-bit fields are then spread to bytes.  Missing
+// flags are summed in byte counters (at most 255 rows), drained into 16-bit
+// pair counters, and added to the output with one atomic per person per row
+// block.
+//
+// The stage writes four bytes per byte read, so its speed is set by how the
+// stores coalesce.  Vector path: a wave owns 1024 packed bytes (4096 people)
+// of each row; lane l decodes dwords l, l+64, l+128, l+192 of that span, so
+// every load instruction reads 256 contiguous bytes and every store writes
+// 1 KB contiguous (full 128-byte lines from one instruction).  Addresses are
+// a per-row buffer resource (uniform base) plus a 32-bit lane offset; lanes
+// past the row end read zeros and their stores are dropped by the buffer
+// range check.  Stores are non-temporal: the tiles are not re-read by this
+// kernel.
 // ---------------------------------------------------------------------------
-constexpr int kUnpackRowChunk = 256;
+constexpr int kUnpackBatch = 8;          // rows loaded before any decode
+constexpr int kUnpackBlock = 256;        // 4 waves
+constexpr int kUnpackRows = 512;         // rows per workgroup (<= 65535 for the pair counters)
+constexpr int kUnpackBytesRows = 2048;   // rows per workgroup of the byte-wise path
+constexpr int kRsrcWord3 = 0x00020000;   // gfx9 raw buffer descriptor: 32-bit data format
+constexpr int kStoreNT = 2;              // buffer store cache policy: non-temporal
 
-__global__ __launch_bounds__(256) void k_unpack(const uint8_t *__restrict__ rows, uint64_t stride,
-                                                uint64_t num_rows, uint64_t num_people,
-                                                const uint8_t *__restrict__ flip,
-                                                uint8_t *__restrict__ dosage,
-                                                uint32_t *__restrict__ missing) {
-    const uint64_t col = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t bpv = (num_people + 3) / 4;
-    if (col >= bpv) return;
-    const uint64_t r0 = (uint64_t)blockIdx.y * kUnpackRowChunk;
-    const uint64_t r1 = r0 + kUnpackRowChunk < num_rows ? r0 + kUnpackRowChunk : num_rows;
-    const uint64_t p0 = col * 4;
-    const bool full = (p0 + 4 <= num_people) && ((num_people & 3) == 0);
-    uint32_t miss = 0;  // four 8-bit counters (chunk <= 255 rows per flush)
-    uint32_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
-    for (uint64_t r = r0; r < r1; ++r) {
-        uint32_t b = rows[r * stride + col];
-        if (flip && flip[r]) b = flip_byte(b);
-        // spread the four 2-bit codes into four bytes, then map code -> tile value
-        const uint32_t x = (b & 0x03u) | ((b & 0x0Cu) << 6) | ((b & 0x30u) << 12) | ((b & 0xC0u) << 18);
-        const uint32_t t = __builtin_amdgcn_perm(0u, 0x02010300u, x);
-        // missing flag per byte: tile value 3
-        const uint32_t is3 = t & (t >> 1) & 0x01010101u;
-        miss += is3;
-        if ((r - r0) == 254) {  // drain byte counters before they can wrap
-            m0 += miss & 0xFF; m1 += (miss >> 8) & 0xFF; m2 += (miss >> 16) & 0xFF; m3 += miss >> 24;
-            miss = 0;
+__device__ __forceinline__ uint32_t spread_fields(uint32_t byte4) {
+    // four 2-bit fields of the low byte -> four bytes
+    uint32_t z = (byte4 | (byte4 << 12)) & 0x000F000Fu;
+    return (z | (z << 6)) & 0x03030303u;
+}
+
+// Tile values of one packed dword (16 people): o[k] byte t = person 4k+t.
+// fm = 0 (A2 dosage) or ~0 (flipped row, A1 dosage), used as data so that a
+// flag still in flight is only waited for where it is consumed.
+__device__ __forceinline__ void decode4(uint32_t w, uint32_t fm, uint32_t (&o)[4], uint32_t (&cnt8)[4]) {
+    const uint32_t L = w & 0x55555555u, H = (w >> 1) & 0x55555555u;
+    const uint32_t B1 = ((L & ~fm) | (~H & fm)) & 0x55555555u;
+    const uint32_t T = (L ^ H) | (B1 << 1);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t z = spread_fields((T >> (8 * k)) & 0xFFu);
+        o[k] = z;
+        cnt8[k] += z & (z >> 1) & 0x01010101u;
+    }
+}
+
+// Vector path.  Requires num_people % 16 == 0, 4-byte aligned rows and
+// stride, 16-byte aligned dosage, 8-byte aligned flip (or NULL) and
+// num_people < 2^32; num_dwords = num_people / 16.
+__global__ __launch_bounds__(kUnpackBlock) void k_unpack(const uint8_t *__restrict__ rows, uint64_t stride,
+                                                         uint64_t num_rows, uint64_t num_people,
+                                                         uint64_t num_dwords, const uint8_t *__restrict__ flip,
+                                                         uint8_t *__restrict__ dosage,
+                                                         uint32_t *__restrict__ missing) {
+    constexpr int R = kUnpackBatch, BS = kUnpackBlock;
+    __shared__ uint32_t lcnt[32 * BS];   // 16-bit pair counters, [slot][thread]
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave = (uint64_t)blockIdx.x * (BS / 64) + (threadIdx.x >> 6);
+    if (wave * 256 >= num_dwords) return;   // whole wave past the row end; no barriers below
+    const uint32_t j0 = (uint32_t)(wave * 256) + lane;
+    // Full per-lane byte offsets kept in VGPRs (opaque to constant folding),
+    // so the range check sees the whole offset.
+    uint32_t ioff[4], ooff[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        ioff[e] = 4u * j0 + 256u * e;
+        ooff[e] = 16u * j0 + 1024u * e;
+        asm volatile("" : "+v"(ioff[e]), "+v"(ooff[e]));
+    }
+    const uint32_t in_bytes = (uint32_t)(num_dwords * 4), out_bytes = (uint32_t)num_people;
+    const uint64_t r0 = (uint64_t)blockIdx.y * kUnpackRows;
+    const uint64_t r1 = r0 + kUnpackRows < num_rows ? r0 + kUnpackRows : num_rows;
+    uint32_t cnt8[4][4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) cnt8[e][k] = 0;
+#pragma unroll
+    for (int k = 0; k < 32; ++k) lcnt[k * BS + threadIdx.x] = 0;
+    auto drain = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                lcnt[(8 * e + 2 * k) * BS + threadIdx.x] += cnt8[e][k] & 0x00FF00FFu;
+                lcnt[(8 * e + 2 * k + 1) * BS + threadIdx.x] += (cnt8[e][k] >> 8) & 0x00FF00FFu;
+                cnt8[e][k] = 0;
+            }
+    };
+    auto load = [&](uint32_t (&w)[4], uint64_t r) __attribute__((always_inline)) {
+        const __amdgpu_buffer_rsrc_t ri =
+            __builtin_amdgcn_make_buffer_rsrc((void *)(rows + r * stride), 0, in_bytes, kRsrcWord3);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) w[e] = __builtin_amdgcn_raw_buffer_load_b32(ri, ioff[e], 0, 0);
+    };
+    auto row = [&](const uint32_t (&w)[4], uint32_t fm, uint64_t r) __attribute__((always_inline)) {
+        const __amdgpu_buffer_rsrc_t ro =
+            __builtin_amdgcn_make_buffer_rsrc(dosage + r * num_people, 0, out_bytes, kRsrcWord3);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            uint32_t o[4];
+            decode4(w[e], fm, o, cnt8[e]);
+            __builtin_amdgcn_raw_buffer_store_b128(u32x4_t{o[0], o[1], o[2], o[3]}, ro, ooff[e], 0, kStoreNT);
         }
-        uint8_t *dst = dosage + r * num_people + p0;
-        if (full) {
-            *(uint32_t *)dst = t;
-        } else {
-            for (int j = 0; j < 4; ++j)
-                if (p0 + j < num_people) dst[j] = (uint8_t)(t >> (8 * j));
+    };
+    uint32_t since = 0;
+    uint64_t r = r0;
+    for (; r + R <= r1; r += R) {
+        uint32_t w[R][4];
+#pragma unroll
+        for (int i = 0; i < R; ++i) load(w[i], r + i);
+        // flip flags of the batch: one uniform 8-byte load (r % 8 == 0)
+        const uint64_t fw = flip ? *(const uint64_t *)(flip + r) : 0;
+        __builtin_amdgcn_sched_barrier(0);   // all loads of the batch issue before its first store
+#pragma unroll
+        for (int i = 0; i < R; ++i) row(w[i], 0u - (uint32_t)(((fw >> (8 * i)) & 0xFFu) != 0), r + i);
+        since += R;
+        if (since + R > 255) {
+            drain();
+            since = 0;
         }
     }
-    m0 += miss & 0xFF; m1 += (miss >> 8) & 0xFF; m2 += (miss >> 16) & 0xFF; m3 += miss >> 24;
-    if (m0 && p0 + 0 < num_people) atomicAdd(&missing[p0 + 0], m0);
-    if (m1 && p0 + 1 < num_people) atomicAdd(&missing[p0 + 1], m1);
-    if (m2 && p0 + 2 < num_people) atomicAdd(&missing[p0 + 2], m2);
-    if (m3 && p0 + 3 < num_people) atomicAdd(&missing[p0 + 3], m3);
+    for (; r < r1; ++r) {   // fewer than R rows left
+        uint32_t w[4];
+        load(w, r);
+        row(w, (flip && flip[r]) ? ~0u : 0u, r);
+    }
+    drain();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const uint32_t j = j0 + 64u * e;
+        if (j >= num_dwords) continue;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const uint32_t c = (lcnt[(8 * e + 2 * k + (t & 1)) * BS + threadIdx.x] >> (16 * (t >> 1))) & 0xFFFFu;
+                if (c) atomicAdd(&missing[16u * j + 4 * k + t], c);
+            }
+    }
+}
+
+// Generic byte-wise path (any num_people, stride and alignment): a thread
+// owns 16 packed bytes (64 people) of each row.
+__device__ __forceinline__ void decode16(u32x4_t w, uint32_t fm, uint32_t (&out)[16], uint32_t (&cnt8)[16]) {
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        uint32_t o[4], c[4] = {cnt8[4 * d], cnt8[4 * d + 1], cnt8[4 * d + 2], cnt8[4 * d + 3]};
+        decode4(w[d], fm, o, c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            out[4 * d + k] = o[k];
+            cnt8[4 * d + k] = c[k];
+        }
+    }
+}
+
+// Byte counters (at most 255 rows) -> 16-bit pair counters: cnt16[2k] holds
+// persons 4k (low half) and 4k+2, cnt16[2k+1] persons 4k+1 and 4k+3.
+__device__ __forceinline__ void drain_counts(uint32_t (&cnt8)[16], uint32_t (&cnt16)[32]) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        cnt16[2 * k] += cnt8[k] & 0x00FF00FFu;
+        cnt16[2 * k + 1] += (cnt8[k] >> 8) & 0x00FF00FFu;
+        cnt8[k] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_unpack_bytes(const uint8_t *__restrict__ rows, uint64_t stride,
+                                                      uint64_t num_rows, uint64_t num_people,
+                                                      const uint8_t *__restrict__ flip,
+                                                      uint8_t *__restrict__ dosage,
+                                                      uint32_t *__restrict__ missing) {
+    const uint64_t chunk = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t bpv = (num_people + 3) / 4;
+    const uint64_t col = chunk * 16, p0 = col * 4;
+    if (col >= bpv) return;
+    const uint64_t r0 = (uint64_t)blockIdx.y * kUnpackBytesRows;
+    const uint64_t r1 = r0 + kUnpackBytesRows < num_rows ? r0 + kUnpackBytesRows : num_rows;
+    uint32_t cnt8[16], cnt16[32], out[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) cnt8[k] = 0;
+#pragma unroll
+    for (int k = 0; k < 32; ++k) cnt16[k] = 0;
+    uint32_t since = 0;
+    for (uint64_t r = r0; r < r1; ++r) {
+        u32x4_t w;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            uint32_t x = 0;
+#pragma unroll
+            for (int bb = 0; bb < 4; ++bb) {
+                const uint64_t o = col + 4 * d + bb;
+                // past the row end: code 01 (missing), never stored or counted
+                x |= (o < bpv ? (uint32_t)rows[r * stride + o] : 0x55u) << (8 * bb);
+            }
+            w[d] = x;
+        }
+        decode16(w, (flip && flip[r]) ? ~0u : 0u, out, cnt8);
+        uint8_t *dst = dosage + r * num_people + p0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                if (p0 + 4 * k + t < num_people) dst[4 * k + t] = (uint8_t)(out[k] >> (8 * t));
+        if (++since == 255) {
+            drain_counts(cnt8, cnt16);
+            since = 0;
+        }
+    }
+    drain_counts(cnt8, cnt16);
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t c = (cnt16[2 * k + (t & 1)] >> (16 * (t >> 1))) & 0xFFFFu;
+            if (c && p0 + 4 * k + t < num_people) atomicAdd(&missing[p0 + 4 * k + t], c);
+        }
+}
+
+static bool unpack_vector_path(const uint8_t *rows, uint64_t stride, uint64_t num_people, const uint8_t *flip,
+                        const uint8_t *dosage) {
+    const uintptr_t a = reinterpret_cast<uintptr_t>(rows), d = reinterpret_cast<uintptr_t>(dosage),
+                    f = reinterpret_cast<uintptr_t>(flip);
+    return num_people % 16 == 0 && num_people < (1ull << 32) && ((a | stride) & 3u) == 0 && (d & 15u) == 0 &&
+           (f & 7u) == 0;
 }
 
 hipError_t launch_unpack(const uint8_t *rows, uint64_t stride, uint64_t num_rows,
@@ -133,9 +315,18 @@ hipError_t launch_unpack(const uint8_t *rows, uint64_t stride, uint64_t num_rows
                          uint32_t *missing, hipStream_t stream) {
     const uint64_t bpv = (num_people + 3) / 4;
     if (num_rows == 0 || bpv == 0) return hipSuccess;
-    dim3 grid((unsigned)((bpv + 255) / 256), (unsigned)((num_rows + kUnpackRowChunk - 1) / kUnpackRowChunk));
-    hipLaunchKernelGGL(k_unpack, grid, dim3(256), 0, stream, rows, stride, num_rows, num_people,
-                       flip, dosage, missing);
+    if (unpack_vector_path(rows, stride, num_people, flip, dosage)) {
+        const uint64_t nd = num_people / 16;
+        const uint64_t per_block = 256 * (kUnpackBlock / 64);
+        hipLaunchKernelGGL(k_unpack, dim3((unsigned)((nd + per_block - 1) / per_block),
+                                          (unsigned)((num_rows + kUnpackRows - 1) / kUnpackRows)),
+                           dim3(kUnpackBlock), 0, stream, rows, stride, num_rows, num_people, nd, flip, dosage,
+                           missing);
+    } else {
+        hipLaunchKernelGGL(k_unpack_bytes, dim3((unsigned)(((bpv + 15) / 16 + 255) / 256),
+                                                (unsigned)((num_rows + kUnpackBytesRows - 1) / kUnpackBytesRows)),
+                           dim3(256), 0, stream, rows, stride, num_rows, num_people, flip, dosage, missing);
+    }
     return hipGetLastError();
 }
 

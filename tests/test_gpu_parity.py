@@ -70,22 +70,25 @@ def test_synth_device_matches_cpu_generator(orc):
     assert np.array_equal(got, want)
 
 
-@pytest.mark.parametrize("N", [64, 1001, 4099])
-def test_unpack_bit_exact(orc, N):
-    _flush(f"unpack N={N}")
+@pytest.mark.parametrize("N,V,with_flip", [(64, 517, True), (1001, 517, True), (4099, 517, True),
+                                           (4112, 517, True), (4112, 517, False), (128, 5000, True)])
+def test_unpack_bit_exact(orc, N, V, with_flip):
+    """Vector path (N a multiple of 16; 4112 people = one full wave span plus
+    16 people), byte path (N odd), several row blocks (V > 512), row counts
+    that are not a multiple of the 8-row batch, no flip array."""
+    _flush(f"unpack N={N} V={V} flip={with_flip}")
     rng = np.random.default_rng(N)
-    V = 517
     dos = rng.choice([0, 1, 2, -1], size=(V, N), p=[0.45, 0.3, 0.2, 0.05])
     dos[3] = -1          # all missing
     dos[4] = 2           # all hom A2
     rows = pack_bed(dos)
-    flip = (rng.random(V) < 0.5).astype(np.uint8)
+    flip = (rng.random(V) < 0.5).astype(np.uint8) if with_flip else np.zeros(V, np.uint8)
     d_rows, d_flip = _dev(rows), _dev(flip)
     d_out = torch.zeros(V * N, dtype=torch.uint8, device="cuda")
     d_miss = torch.zeros(N, dtype=torch.int32, device="cuda")
     torch.cuda.synchronize()
     _lib.check(_lib.lib().gnm_unpack_device(ctypes.c_void_p(d_rows.data_ptr()), rows.shape[1], V, N,
-                                            ctypes.c_void_p(d_flip.data_ptr()),
+                                            ctypes.c_void_p(d_flip.data_ptr() if with_flip else None),
                                             ctypes.c_void_p(d_out.data_ptr()),
                                             ctypes.c_void_p(d_miss.data_ptr()), None))
     torch.cuda.synchronize()
