@@ -1,0 +1,191 @@
+"""Differentiable path on the GPU: dense forward and adjoint against the FP64
+oracle, and the torch.autograd.Function against central finite differences
+and the FP64 CPU gradient.
+
+Error bounds follow from the split-limb construction: a value x in a
+(block, column) cell with maximum magnitude m is represented as an integer
+multiple of a power-of-two unit u with m <= (2^(8L) - 2^(8(L-1))) u, so
+|x - x_hat| <= u / 2 <= m / (2^(8L+1) - 2^(8L-7)).  Dosages are exact, and
+products and partial sums inside a block are exact (integers below 2^24), so
+the contraction error is the sum of those representation errors weighted by
+the dosages.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from gnomon_amd import _lib
+from gnomon_amd import autograd as ag
+from helpers import pack_bed
+
+pytestmark = pytest.mark.gpu
+
+BLOCK = 8192   # rows (forward: variants; adjoint: people) per scale block
+LIMBS = {"bf16x3": 3, "bf16x2": 2}
+
+
+def _flush(msg):
+    print(f"\n[case] {msg}", flush=True)
+
+
+def _data(N, V, K, seed, zero_frac=0.3):
+    rng = np.random.default_rng(seed)
+    dos = rng.choice([0, 1, 2, -1], size=(V, N), p=[0.45, 0.35, 0.17, 0.03])
+    rows = pack_bed(dos)
+    flip = (rng.random(V) < 0.5).astype(np.uint8)
+    W = rng.laplace(0, 1e-3, (V, K)).astype(np.float32)
+    W[rng.random((V, K)) < zero_frac] = 0.0
+    return dos, rows, flip, W
+
+
+def _effect_dosage(dos, flip):
+    """[V, N] effect-allele dosage, missing -> 0 (float64)."""
+    g = np.where(dos < 0, 0, dos).astype(np.float64)
+    g = np.where((flip[:, None] == 1) & (dos >= 0), 2.0 - g, g)
+    return g
+
+
+def _limb_bound(G, X, L, axis_rows):
+    """max error of sum over rows r of G[r, :] * X[r, k] when X is limb-split
+    per BLOCK rows and column: sum_blk (sum_{r in blk} G[r, i]) * m_blk,k / d."""
+    d = 2.0 ** (8 * L + 1) - 2.0 ** (8 * L - 7)
+    R = X.shape[0]
+    out = 0.0
+    for b0 in range(0, R, BLOCK):
+        m = np.abs(X[b0:b0 + BLOCK]).max(0)          # [K]
+        gsum = G[b0:b0 + BLOCK].sum(0)               # [I]
+        out = out + np.outer(gsum, m) / d
+    return out
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16x2"])
+def test_dense_forward_matches_fp64(orc, precision):
+    N, V, K = 3001, 9000, 16          # two variant scale blocks, ragged N and V
+    _flush(f"dense forward {precision} N={N} V={V} K={K}")
+    dos, rows, flip, W = _data(N, V, K, 11)
+    S64, M64 = orc.dense_forward_fp64(rows, N, flip, W)
+    S, miss = ag.dense_forward(_dev(W), _dev(rows), _dev(flip), N, precision)
+    torch.cuda.synchronize()
+    S, miss = S.cpu().numpy(), miss.cpu().numpy()
+    assert np.array_equal(miss.astype(np.int64), M64.astype(np.int64))
+    err = np.abs(S - S64)
+    if precision == "fp32":
+        assert err.max() <= 1e-12 * np.abs(S64).max()
+    else:
+        G = _effect_dosage(dos, flip)                 # [V, N]
+        bound = _limb_bound(G, W.astype(np.float64), LIMBS[precision], "variants")
+        assert np.all(err <= bound + 1e-12 * np.abs(S64).max()), (err - bound).max()
+        if precision == "bf16x3":
+            assert err.max() / np.abs(S64).max() <= 1e-6
+
+
+@pytest.mark.parametrize("precision,K", [("bf16x3", 16), ("bf16x3", 40), ("bf16x2", 16)])
+def test_adjoint_matches_fp64(orc, precision, K):
+    N, V = 9000, 4100                  # two person scale blocks; last workgroup partial
+    _flush(f"adjoint {precision} N={N} V={V} K={K}")
+    dos, rows, flip, _ = _data(N, V, 1, 12)
+    rng = np.random.default_rng(13)
+    R = rng.normal(0, 1, (N, K)).astype(np.float32)
+    R[:, 3] *= 1e-6                    # a small column: scales are per column
+    dW64 = orc.adjoint_fp64(rows, N, flip, R.astype(np.float64))
+    dW = ag.dense_adjoint(_dev(R), _dev(rows), _dev(flip), V, precision)
+    torch.cuda.synchronize()
+    dW = dW.cpu().numpy().astype(np.float64)
+    G = _effect_dosage(dos, flip).T                   # [N, V]
+    bound = _limb_bound(G, R.astype(np.float64), LIMBS[precision], "people")
+    err = np.abs(dW - dW64)
+    assert np.all(err <= bound + 2.0 ** -23 * np.abs(dW64) + 1e-30), (err - bound).max()
+
+
+def test_adjoint_rejects_too_many_scores():
+    N, V, K = 256, 64, 65
+    rows = np.zeros((V, N // 4), np.uint8)
+    R = torch.zeros(N, K, dtype=torch.float32, device="cuda")
+    with pytest.raises(_lib.GnmError):
+        ag.dense_adjoint(R, _dev(rows), None, V, "bf16x3")
+
+
+def _ridge_problem(N, V, K, seed):
+    dos, rows, flip, W = _data(N, V, K, seed, zero_frac=0.0)
+    rng = np.random.default_rng(seed + 1)
+    G = _effect_dosage(dos, flip)                     # [V, N]
+    w_star = rng.normal(0, 0.02, V)
+    y = G.T @ w_star + rng.normal(0, 1, N)
+    return dos, rows, flip, W, G, y
+
+
+def test_autograd_matches_finite_differences_and_fp64_gradient(orc):
+    """Ridge-penalised MSE L(W) = sum_k mean_p (S[p,k] - y[p])^2 + lam ||W||^2
+    with S from the split-BF16 forward; gradient through the adjoint."""
+    N, V, K, lam = 2000, 3000, 4, 1e-3
+    _flush(f"autograd FD N={N} V={V} K={K}")
+    dos, rows, flip, W0, G, y = _ridge_problem(N, V, K, 21)
+    d_rows, d_flip = _dev(rows), _dev(flip)
+    y_t = torch.from_numpy(y).cuda()
+
+    W = torch.from_numpy(W0).cuda().requires_grad_(True)
+    S = ag.polygenic_score(W, d_rows, d_flip, N, "bf16x3")
+    loss = ((S - y_t[:, None]) ** 2).mean(0).sum() + lam * (W.double() ** 2).sum()
+    loss.backward()
+    g_auto = W.grad.detach().cpu().numpy().astype(np.float64)
+
+    # FP64 CPU gradient: dL/dW = G (2/N)(S64 - y) + 2 lam W
+    S64, _ = orc.dense_forward_fp64(rows, N, flip, W0)
+    R64 = 2.0 / N * (S64 - y[:, None])
+    g64 = orc.adjoint_fp64(rows, N, flip, R64) + 2 * lam * W0.astype(np.float64)
+    scale = np.abs(g64).max()
+    assert np.abs(g_auto - g64).max() <= 1e-5 * scale
+
+    # central finite differences of the FP64 loss on 100 random weights, the
+    # loss evaluated with the exact-product FP32-weight forward
+    def loss_at(Wnp):
+        Sx, _ = ag.dense_forward(torch.from_numpy(Wnp).cuda(), d_rows, d_flip, N, "fp32")
+        Sx = Sx.cpu().numpy()
+        return float(((Sx - y[:, None]) ** 2).mean(0).sum() + lam * (Wnp.astype(np.float64) ** 2).sum())
+
+    rng = np.random.default_rng(5)
+    picks = rng.choice(V * K, 100, replace=False)
+    h = 2.0 ** -12                       # exact in f32 next to weights ~1e-3
+    fd = np.empty(100)
+    for i, idx in enumerate(picks):
+        v, k = divmod(int(idx), K)
+        Wp, Wm = W0.copy(), W0.copy()
+        Wp[v, k] += h
+        Wm[v, k] -= h
+        hp = float(Wp[v, k]) - float(W0[v, k])
+        hm = float(W0[v, k]) - float(Wm[v, k])
+        fd[i] = (loss_at(Wp) - loss_at(Wm)) / (hp + hm)
+    ga = g_auto.reshape(-1)[picks]
+    g6 = g64.reshape(-1)[picks]
+    assert np.abs(fd - g6).max() <= 1e-7 * scale       # the FD itself is sound
+    assert np.abs(ga - fd).max() <= 1e-5 * scale
+
+
+def test_gradient_steps_reduce_the_loss():
+    """20 plain gradient steps on the ridge problem with step 1/lambda_max of
+    the Hessian (2/N) G G^T + 2 lam I: the loss decreases at every step."""
+    N, V, K, lam = 2000, 3000, 2, 1e-3
+    _flush("autograd 20 steps")
+    _, rows, flip, W0, G, y = _ridge_problem(N, V, K, 31)
+    lam_max = 2.0 / N * np.linalg.norm(G, 2) ** 2 + 2 * lam
+    d_rows, d_flip = _dev(rows), _dev(flip)
+    y_t = torch.from_numpy(y).cuda()
+    W = torch.zeros(V, K, dtype=torch.float32, device="cuda", requires_grad=True)
+    opt = torch.optim.SGD([W], lr=1.0 / lam_max)
+    losses = []
+    for _ in range(20):
+        opt.zero_grad()
+        S = ag.polygenic_score(W, d_rows, d_flip, N, "bf16x3")
+        loss = ((S - y_t[:, None]) ** 2).mean(0).sum() + lam * (W.double() ** 2).sum()
+        loss.backward()
+        opt.step()
+        losses.append(float(loss.detach()))
+    assert all(np.isfinite(losses))
+    assert all(b <= a * (1 + 1e-9) for a, b in zip(losses, losses[1:]))
+    assert losses[-1] < losses[0]
