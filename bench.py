@@ -24,6 +24,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# arithmetic of each path: limb type -> exact block accumulator -> cross-block reduction
+DTYPE = {"fp32": "f32*{0,1,2}->f64", "bf16x2": "bf16x2->f32->f64", "bf16x3": "bf16x3->f32->f64",
+         "i8x3": "i8x3->i32->f64", "i8x4": "i8x4->i32->f64"}
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
@@ -116,13 +119,200 @@ def measure_unpack(d_bed, N, V, bpv, stride, reps, workload):
             "traffic": traffic, "traffic_source": src}
 
 
+def _errors(S, S64):
+    """Per-score error metrics against an FP64 reference [N, K]: (i) max
+    relative error over people with |S64| >= 1e-3 max|S64|, (ii) normwise
+    max|S - S64| / max|S64|."""
+    import numpy as np
+
+    K = S64.shape[1]
+    rel, nw = np.zeros(K), np.zeros(K)
+    for k in range(K):
+        ref = S64[:, k]
+        m = np.abs(ref).max()
+        if m == 0:
+            continue
+        sel = np.abs(ref) >= 1e-3 * m
+        rel[k] = (np.abs(S[sel, k] - ref[sel]) / np.abs(ref[sel])).max()
+        nw[k] = np.abs(S[:, k] - ref).max() / m
+    return rel, nw
+
+
+def _avg_rule_fraction(S, C, S64, C64, counts, tol=1e-6):
+    """Fraction of (person, score) cells whose AVG = sum / (variants - missing)
+    meets the reference's GPU/CPU agreement rule (score/tests/gpu_tests.rs:134-137):
+    |a - b| <= tol or |a - b| / max(|a|, |b|, 1) <= tol."""
+    import numpy as np
+
+    den, den64 = counts[None, :].astype(np.float64) - C, counts[None, :].astype(np.float64) - C64
+    a = np.where(den > 0, S / np.maximum(den, 1), 0.0)
+    b = np.where(den64 > 0, S64 / np.maximum(den64, 1), 0.0)
+    d = np.abs(a - b)
+    ok = (d <= tol) | (d / np.maximum(np.maximum(np.abs(a), np.abs(b)), 1.0) <= tol)
+    return float(ok.mean())
+
+
+def run_precision_report(w, csr, d_bed, stride, args):
+    """Per-score error of each precision at full size: against the CPU FP64
+    oracle on the first 128 people (all variants), and against the plain FP32
+    path (exact products, FP64 accumulation) on all people."""
+    import numpy as np
+
+    from gnomon_amd.pipeline import ScoringContext
+    from oracle import oracle as orc
+
+    N, V, K = w.num_people, w.num_variants, w.num_scores
+    sample = 128
+    t0 = time.time()
+    rows = d_bed.view(V, stride)[:, :sample // 4].cpu().numpy()
+    S64, C64 = orc.score_fp64(rows, csr, sample)
+    log(f"FP64 oracle on {sample} people x {V} variants x {K} scores ({time.time() - t0:.1f}s)")
+    out, full = {}, {}
+    for prec in ("fp32", "bf16x3", "bf16x2", "i8x4", "i8x3"):
+        with ScoringContext(csr, N, precision=prec, force=True) as c:
+            c.score_rows_device(d_bed.data_ptr(), stride, 0, V)
+            S, C = c.finish()
+        full[prec] = S
+        rel, nw = _errors(S[:sample], S64)
+        out[prec] = {"vs_fp64_oracle_sample_people": sample,
+                     "max_rel_err_per_score": [float(f"{x:.3e}") for x in rel],
+                     "max_rel_err": float(rel.max()), "max_normwise_err": float(nw.max()),
+                     "counts_exact": bool(np.array_equal(C[:sample], C64)),
+                     "reference_avg_rule_pass_fraction": _avg_rule_fraction(S[:sample], C[:sample], S64, C64,
+                                                                             csr.score_variant_counts)}
+        log(f"[{prec}] vs FP64 oracle: max rel {rel.max():.3e}, normwise {nw.max():.3e}")
+    for prec in ("bf16x3", "bf16x2", "i8x4", "i8x3"):
+        rel, nw = _errors(full[prec], full["fp32"])
+        out[prec]["vs_fp32_path_all_people"] = {"max_rel_err": float(rel.max()), "max_normwise_err": float(nw.max())}
+    return out
+
+
+def run_differentiation(w, args):
+    """Ridge-penalised regression through the differentiable path at 50,000
+    people x 500,000 variants x 16 scores: y = G w* + N(0, 1),
+    L(W) = sum_k mean_p (S[p,k] - y[p])^2 + lam ||W||^2 with S = G^T W from the
+    split-BF16 forward and dL/dW from the adjoint kernel.  Times 20 gradient
+    steps; checks the gradient against central finite differences on 100
+    random weights and against the FP64 CPU gradient of those weights."""
+    import numpy as np
+    import torch
+
+    from gnomon_amd import autograd as ag
+    from gnomon_amd import synth
+    from oracle import oracle as orc
+
+    N, V, K, lam = w.num_people, w.num_variants, w.num_scores, 1e-3
+    bpv = w.bytes_per_variant
+    stride = (bpv + 255) // 256 * 256
+    rng = np.random.default_rng(w.seed + 7)
+    d_bed = torch.empty(V, stride, dtype=torch.uint8, device="cuda")
+    synth.synth_rows_device(w, d_bed.data_ptr(), stride, 0, V)
+    flip_np = (rng.random(V) < 0.5).astype(np.uint8)
+    d_flip = torch.from_numpy(flip_np).cuda()
+    w_star = torch.from_numpy(rng.normal(0, 0.02, (V, 1)).astype(np.float32)).cuda()
+    Sy, _ = ag.dense_forward(w_star, d_bed, d_flip, N, "fp32")
+    y = Sy[:, 0] + torch.from_numpy(rng.normal(0, 1, N)).cuda()
+    W0 = rng.laplace(0, 1e-3, (V, K)).astype(np.float32)
+    torch.cuda.synchronize()
+    log(f"differentiation: N={N} V={V} K={K}, .bed {V * bpv / 1e9:.2f} GB on device")
+
+    def loss_fn(S, W):
+        return ((S - y[:, None]) ** 2).mean(0).sum() + lam * (W.double() ** 2).sum()
+
+    # gradient at W0: autograd, FP64 CPU, central FD
+    W = torch.from_numpy(W0).cuda().requires_grad_(True)
+    loss = loss_fn(ag.polygenic_score(W, d_bed, d_flip, N, "bf16x3"), W)
+    loss.backward()
+    g_auto = W.grad.detach().cpu().numpy().astype(np.float64)
+    S_ref, _ = ag.dense_forward(torch.from_numpy(W0).cuda(), d_bed, d_flip, N, "fp32")
+    R64 = (2.0 / N * (S_ref - y[:, None])).cpu().numpy()
+    picks = rng.choice(V * K, 100, replace=False)
+    pv = np.unique(picks // K)
+    rows_sub = d_bed[torch.from_numpy(pv).cuda()][:, :bpv].cpu().numpy()
+    g64_rows = orc.adjoint_fp64(rows_sub, N, flip_np[pv], R64)
+    g64 = np.array([g64_rows[np.searchsorted(pv, i // K), i % K] for i in picks]) + 2 * lam * W0.reshape(-1)[picks]
+
+    def loss_at(Wnp):
+        Sx, _ = ag.dense_forward(torch.from_numpy(Wnp).cuda(), d_bed, d_flip, N, "fp32")
+        return float(loss_fn(Sx, torch.from_numpy(Wnp)))
+
+    h = 2.0 ** -12
+    fd = np.empty(len(picks))
+    t0 = time.time()
+    for i, idx in enumerate(picks):
+        v, k = divmod(int(idx), K)
+        Wp, Wm = W0.copy(), W0.copy()
+        Wp[v, k] += h
+        Wm[v, k] -= h
+        fd[i] = (loss_at(Wp) - loss_at(Wm)) / ((float(Wp[v, k]) - float(W0[v, k])) + (float(W0[v, k]) - float(Wm[v, k])))
+    ga = g_auto.reshape(-1)[picks]
+    scale = np.abs(g64).max()
+    fd_err = float(np.abs(ga - fd).max() / scale)
+    g64_err = float(np.abs(ga - g64).max() / scale)
+    log(f"gradient check ({time.time() - t0:.1f}s FD): max |g - FD| / max|g| = {fd_err:.3e}, "
+        f"vs FP64 CPU gradient {g64_err:.3e}")
+
+    # step size 1/lambda_max of the Hessian (2/N) G G^T + 2 lam I (power iteration)
+    u = torch.from_numpy(rng.normal(0, 1, (V, 1)).astype(np.float32)).cuda()
+    lam_max = 0.0
+    for _ in range(12):
+        s, _ = ag.dense_forward(u, d_bed, d_flip, N, "fp32")
+        hu = ag.dense_adjoint((2.0 / N) * s, d_bed, d_flip, V, "bf16x3").double() + 2 * lam * u.double()
+        lam_max = float(hu.norm() / u.double().norm())
+        u = (hu / hu.norm()).float()
+    Wt = torch.zeros(V, K, dtype=torch.float32, device="cuda", requires_grad=True)
+    opt = torch.optim.SGD([Wt], lr=1.0 / lam_max)
+    for _ in range(2):    # warm-up steps on a throw-away copy
+        Wc = Wt.detach().clone().requires_grad_(True)
+        loss_fn(ag.polygenic_score(Wc, d_bed, d_flip, N, "bf16x3"), Wc).backward()
+    torch.cuda.synchronize()
+    losses = []
+    t0 = time.perf_counter()
+    for _ in range(20):
+        opt.zero_grad()
+        loss = loss_fn(ag.polygenic_score(Wt, d_bed, d_flip, N, "bf16x3"), Wt)
+        loss.backward()
+        opt.step()
+        losses.append(loss.detach())
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) * 1e3 / 20
+    losses = [float(x) for x in losses]
+    # kernel-level split of one step
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    R = torch.randn(N, K, device="cuda", dtype=torch.float32)
+    ev[0].record()
+    ag.dense_forward(Wt.detach(), d_bed, d_flip, N, "bf16x3")
+    ev[1].record()
+    ag.dense_adjoint(R, d_bed, d_flip, V, "bf16x3")
+    ev[2].record()
+    torch.cuda.synchronize()
+    fwd_ms, adj_ms = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+    svs = N * V * K
+    log(f"20 steps: {ms_step:.2f} ms/step (forward {fwd_ms:.2f} ms, adjoint {adj_ms:.2f} ms); "
+        f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    out = {
+        "metric": "gradient_steps/s", "value": 1e3 / ms_step, "unit": "steps/s", "n_gpus": 1, "steps": 20,
+        "warmup": 2, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3->f32->f64", "data": "synthetic (device-generated HWE genotypes)",
+        "config": {"workload": w.name, "people": N, "variants": V, "scores": K, "lambda": lam,
+                   "step": "SGD, lr = 1/lambda_max"},
+        "forward_ms": fwd_ms, "adjoint_ms": adj_ms,
+        "forward_svs_per_s": svs / (fwd_ms / 1e3), "adjoint_svs_per_s": svs / (adj_ms / 1e3),
+        "gradient_check": {"weights": len(picks), "max_err_vs_central_fd": fd_err, "max_err_vs_fp64_cpu": g64_err,
+                           "normalised_by": "max |FP64 gradient| over the checked weights"},
+        "loss_first": losses[0], "loss_last": losses[-1],
+        "loss_monotone": all(b <= a * (1 + 1e-9) for a, b in zip(losses, losses[1:])),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="plain_fp32_biobank")
-    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16x2", "bf16x3"])
+    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16x2", "bf16x3", "i8x3", "i8x4"])
     ap.add_argument("--compare", default="fp32", help="second path timed for the report ('' = none)")
     ap.add_argument("--variants", type=int, default=0, help="override the variant count")
     ap.add_argument("--cpu-sample-variants", type=int, default=30000)
@@ -151,6 +341,9 @@ def main():
     w = synth.WORKLOADS[args.workload]
     if args.variants:
         w = w.scaled(num_variants=args.variants)
+    if w.name == "adjoint_16":
+        run_differentiation(w, args)
+        return
     N, V, K = w.num_people, w.num_variants, w.num_scores
     bpv = w.bytes_per_variant
     stride = (bpv + 255) // 256 * 256
@@ -220,13 +413,16 @@ def main():
     # one f32 weight matrix + f64 scores and u32 counts
     alg_bytes = V * bpv + V * K * 4 + N * K * (8 + 4)
     achieved = alg_bytes / (main_ms / 1e3) / 1e9
-    kname = "k_score_mfma" if args.precision != "fp32" else "k_score_fp32"
+    kname = {"fp32": "k_score_fp32", "i8x3": "k_score_i8", "i8x4": "k_score_i8"}.get(args.precision, "k_score_mfma")
     traffic, tsrc = profiled_traffic(kname, w.name, args.precision, alg_bytes, main_ms)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc,
                 "kernel": kname, "kernel_ms": round(main_ms, 4), "algorithmic_bytes": alg_bytes}
 
-    unpack = None if args.no_unpack else measure_unpack(d_bed, N, V, bpv, stride, max(3, min(args.steps, 10)), w.name)
+    precision_report = None
+    if w.name == "precision_64":
+        precision_report = run_precision_report(w, csr, d_bed, stride, args)
+    unpack = None if args.no_unpack or w.name != "plain_fp32_biobank" else measure_unpack(d_bed, N, V, bpv, stride, max(3, min(args.steps, 10)), w.name)
 
     cpu = None
     if not args.no_cpu:
@@ -250,12 +446,12 @@ def main():
         "metric": "sample*variant*score/s", "value": value, "unit": "sample*variant*score/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"fp32": "f32", "bf16x2": "bf16x2->f32->f64", "bf16x3": "bf16x3->f32->f64"}[args.precision],
+        "dtype": DTYPE[args.precision],
         "data": "synthetic (device-generated HWE genotypes, numpy weights)",
         "config": {"workload": w.name, "people": N, "variants": V, "scores": K,
                    "bed_bytes": V * bpv, "precision": args.precision},
         "roofline": roofline, "cpu_baseline": cpu, "compare": compare, "check": check,
-        "unpack_stage": unpack,
+        "unpack_stage": unpack, "precision_report": precision_report,
     }
     print(json.dumps(out), flush=True)
 

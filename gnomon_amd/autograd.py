@@ -22,7 +22,7 @@ import torch
 
 from . import _lib
 
-_PREC = {"fp32": _lib.PRECISION["fp32"], "bf16x2": _lib.PRECISION["bf16x2"], "bf16x3": _lib.PRECISION["bf16x3"]}
+_PREC = {k: _lib.PRECISION[k] for k in ("fp32", "bf16x2", "bf16x3", "i8x3", "i8x4")}
 
 
 def _ptr(t):
@@ -63,7 +63,7 @@ def dense_adjoint(R, bed, flip, num_variants, precision="bf16x3"):
     dW = torch.empty(num_variants, K, dtype=torch.float32, device=R.device)
     _lib.check(_lib.lib().gnm_dense_adjoint_device(
         _ptr(bed), bed.shape[1], num_variants, N, _ptr(flip), _ptr(R), K,
-        _PREC["bf16x3" if precision == "fp32" else precision], _ptr(dW), _stream()))
+        _PREC["bf16x2" if precision == "bf16x2" else "bf16x3"], _ptr(dW), _stream()))
     return dW
 
 
@@ -73,8 +73,8 @@ class PolygenicScore(torch.autograd.Function):
     W: [V, K] float32 CUDA tensor; bed: uint8 CUDA tensor [V, row_stride]
     (packed SNP-major rows); flip: uint8 CUDA tensor [V] (1 = effect allele
     is A1) or None.  Returns S [N, K] float64.  precision selects the forward
-    arithmetic ("bf16x3", "bf16x2" or "fp32"); the adjoint runs split-BF16
-    (bf16x3 unless "bf16x2" is asked for).
+    arithmetic ("bf16x3", "bf16x2", "i8x4", "i8x3" or "fp32"); the adjoint
+    runs split-BF16 (bf16x3 unless "bf16x2" is asked for).
     """
 
     @staticmethod

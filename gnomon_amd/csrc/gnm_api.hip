@@ -169,7 +169,7 @@ gnm_status validate(const gnm_problem *p) {
         for (uint64_t i = 0; i < p->num_people; ++i)
             if (p->output_idx_to_fam_idx[i] >= p->total_people_in_fam)
                 return fail(GNM_ERR_INVALID_ARG, "output_idx_to_fam_idx entry out of range");
-    if (p->precision < GNM_PRECISION_FP32 || p->precision > GNM_PRECISION_AUTO)
+    if (p->precision < GNM_PRECISION_FP32 || p->precision > GNM_PRECISION_I8X4)
         return fail(GNM_ERR_INVALID_ARG, "unknown precision");
     for (int i = 0; i < 4; ++i)
         if (p->reserved[i]) return fail(GNM_ERR_INVALID_ARG, "reserved fields must be zero");
@@ -185,6 +185,26 @@ inline void limb_digits(int64_t Q, uint32_t L, int32_t *out) {
         Q = (Q - d) / 256;
     }
     out[0] = (int32_t)Q;
+}
+
+inline bool is_int8(int precision) {
+    return precision == GNM_PRECISION_I8X3 || precision == GNM_PRECISION_I8X4;
+}
+
+inline uint32_t limbs_of(int precision) {
+    switch (precision) {
+        case GNM_PRECISION_BF16X2: return 2;
+        case GNM_PRECISION_BF16X3: return 3;
+        case GNM_PRECISION_I8X3: return 3;
+        case GNM_PRECISION_I8X4: return 4;
+        default: return 1;
+    }
+}
+
+// largest |Q| the limb digits represent: BF16 limbs allow a top digit up to
+// 255, int8 limbs keep every digit in [-128, 127]
+inline double limb_qmax(uint32_t L, bool int8) {
+    return int8 ? 127.0 * std::ldexp(1.0, 8 * (int)(L - 1)) : std::ldexp(1.0, 8 * (int)L) - std::ldexp(1.0, 8 * (int)(L - 1));
 }
 
 inline uint16_t bf16_of_int(int32_t q) {
@@ -220,7 +240,8 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
     c->V = p->num_variants;
     c->precision = p->precision == GNM_PRECISION_AUTO ? GNM_PRECISION_BF16X3 : p->precision;
     c->device = p->device < 0 ? 0 : p->device;
-    c->limbs = c->precision == GNM_PRECISION_BF16X2 ? 2 : (c->precision == GNM_PRECISION_BF16X3 ? 3 : 1);
+    c->limbs = limbs_of(c->precision);
+    const bool int8 = is_int8(c->precision);
     const uint64_t V = c->V, K = c->K, nnz = p->nnz;
 
     c->identity = true;
@@ -333,6 +354,15 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
     pl.nclass = c->nclass;
     pl.score_class = c->d_score_class;
     pl.limbs = c->limbs;
+    pl.int8 = int8 ? 1u : 0u;
+    if (int8) {
+        std::vector<uint8_t> flipbyte(Rpad, 0);
+        for (uint64_t r = 0; r < R; ++r)
+            if ((flipmask[r >> 5] >> (r & 31)) & 1u) flipbyte[r] = 4;
+        uint8_t *d_flipbyte;
+        if ((st = c->upload(&d_flipbyte, flipbyte)) != GNM_OK) return st;
+        pl.flipbyte = d_flipbyte;
+    }
 
     const bool mfma_weights = c->precision != GNM_PRECISION_FP32;
     c->fp32_count_in_kernel = c->nclass <= (uint32_t)kFp32MaxClasses;
@@ -355,13 +385,14 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
     pl.ctiles = mfma_counts ? (c->nclass + 15) / 16 : 0;
     const uint32_t tiles = pl.wtiles + pl.ctiles;
     if (tiles) {
-        const uint64_t nks = Rpad / kKStep;
+        const uint64_t kstep = int8 ? (uint64_t)kI8KStep : (uint64_t)kKStep;
+        const uint64_t nks = Rpad / kstep;
         const uint64_t nblk = (Rpad + kFlushRows - 1) / kFlushRows;
         std::vector<double> scales(mfma_weights ? nblk * pl.ncol : 1, 0.0);
         // per (scale block, score): u = 2^e with max|w|/u <= 2^(8L) - 2^(8(L-1))
         std::vector<int> uexp(mfma_weights ? nblk * K : 1, 0);
         if (mfma_weights) {
-            const double qmax = std::ldexp(1.0, 8 * L) - std::ldexp(1.0, 8 * (L - 1));
+            const double qmax = limb_qmax(L, int8);
             parallel_for(nblk, [&](uint64_t a, uint64_t b) {
                 for (uint64_t blk = a; blk < b; ++blk) {
                     const uint64_t r0 = blk * kFlushRows, r1 = std::min<uint64_t>(R, r0 + kFlushRows);
@@ -381,6 +412,50 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
                 }
             });
         }
+        // Value of fragment element (row r, column tile*16 + n): a limb digit of
+        // the effect-allele weight, or 1 on count rows of the class column.
+        auto frag_value = [&](uint64_t r, uint32_t tile, uint32_t n, uint64_t blk, int32_t *dig) -> int32_t {
+            if (r >= R) return 0;
+            if (tile < pl.wtiles) {
+                const uint32_t colw = tile * 16 + n;
+                if (colw >= pl.ncol) return 0;
+                const uint32_t limb = colw / (uint32_t)K;
+                const uint64_t k = colw % K;
+                const double w = wrow[r * K + k];
+                if (w == 0.0) return 0;
+                const double Qd = std::nearbyint(std::ldexp(w, -uexp[blk * K + k]));
+                limb_digits((int64_t)Qd, L, dig);
+                return dig[limb];
+            }
+            const uint32_t cl = (tile - pl.wtiles) * 16 + n;
+            if (cl < c->nclass && prog_count_row[r]) {
+                const uint64_t v = prog_var[r];
+                for (uint64_t e = c->row_offsets[v]; e < c->row_offsets[v + 1]; ++e)
+                    if (c->score_class[c->cols[e]] == cl) return 1;
+            }
+            return 0;
+        };
+        if (int8) {
+            // [ks][tile][lane][16 i8]; lane (n = lane%16, g = lane/16) byte b
+            // holds row 64ks + 16g + b of column tile*16 + n.
+            std::vector<uint8_t> frag8(nks * tiles * 64 * 16, 0);
+            parallel_for(nks, [&](uint64_t a, uint64_t b) {
+                std::vector<int32_t> dig(L);
+                for (uint64_t ks = a; ks < b; ++ks) {
+                    const uint64_t blk = ks * kstep / kFlushRows;
+                    for (uint32_t tile = 0; tile < tiles; ++tile)
+                        for (int lane = 0; lane < 64; ++lane)
+                            for (int j = 0; j < 16; ++j) {
+                                const uint64_t r = ks * kstep + (lane >> 4) * 16 + j;
+                                const int32_t v = frag_value(r, tile, lane & 15, blk, dig.data());
+                                frag8[((ks * tiles + tile) * 64 + lane) * 16 + j] = (uint8_t)(int8_t)v;
+                            }
+                }
+            });
+            uint8_t *d_frag8;
+            if ((st = c->upload(&d_frag8, frag8)) != GNM_OK) return st;
+            pl.bfrag = (const uint32_t *)d_frag8;
+        } else {
         // B fragments: [ks][tile][lane][8 bf16]; lane (n = lane%16, g = lane/16)
         // holds rows 32ks + 8g + j (j = 0..7) of column tile*16 + n.
         std::vector<uint16_t> frag(nks * tiles * 64 * 8, 0);
@@ -426,6 +501,7 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
         uint16_t *d_frag;
         if ((st = c->upload(&d_frag, frag)) != GNM_OK) return st;
         pl.bfrag = (const uint32_t *)d_frag;
+        }
         if (mfma_weights) {
             double *d_scales;
             if ((st = c->upload(&d_scales, scales)) != GNM_OK) return st;
@@ -521,6 +597,9 @@ gnm_status score_device_rows(gnm_ctx *c, const uint8_t *d_rows, uint64_t stride,
         if (!c->fp32_count_in_kernel)
             GNM_HIP(launch_score_mfma(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, S, 0,
                                       c->plan.ctiles, identity, c->d_part, c->d_cpart, c->stream));
+    } else if (c->plan.int8) {
+        GNM_HIP(launch_score_i8(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, S, 0,
+                                c->plan.wtiles + c->plan.ctiles, identity, c->d_part, c->d_cpart, c->stream));
     } else {
         GNM_HIP(launch_score_mfma(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, S, 0,
                                   c->plan.wtiles + c->plan.ctiles, identity, c->d_part, c->d_cpart, c->stream));
@@ -845,14 +924,15 @@ gnm_status gnm_dense_forward_device(const uint8_t *d_rows, uint64_t bytes_per_va
         if (V && (!d_rows || !d_weights)) return fail(GNM_ERR_INVALID_ARG, "NULL device pointer");
         if (!d_scores || !d_missing_per_person) return fail(GNM_ERR_INVALID_ARG, "NULL output pointer");
         if (bytes_per_variant < (N + 3) / 4) return fail(GNM_ERR_INVALID_ARG, "bytes_per_variant too small");
-        if (precision < GNM_PRECISION_FP32 || precision > GNM_PRECISION_AUTO)
+        if (precision < GNM_PRECISION_FP32 || precision > GNM_PRECISION_I8X4)
             return fail(GNM_ERR_INVALID_ARG, "unknown precision");
         if (precision == GNM_PRECISION_AUTO) precision = GNM_PRECISION_BF16X3;
         hipStream_t st = (hipStream_t)stream_;
         DenseWorkspace *ws = workspace_for_current_device();
         if (!ws) return fail(GNM_ERR_NO_DEVICE, "no current HIP device");
         std::lock_guard<std::mutex> lk(ws->mu);
-        const uint32_t L = precision == GNM_PRECISION_BF16X2 ? 2 : (precision == GNM_PRECISION_BF16X3 ? 3 : 1);
+        const uint32_t L = limbs_of(precision);
+        const bool int8 = is_int8(precision);
         const uint64_t Rpad = std::max<uint64_t>((V + 63) / 64 * 64, 64);
         const uint32_t Lw = L == 1 ? 0 : L;
         const uint32_t S = precision == GNM_PRECISION_FP32 ? fp32_splits(N, K)
@@ -894,20 +974,32 @@ gnm_status gnm_dense_forward_device(const uint8_t *d_rows, uint64_t bytes_per_va
             pl.wtiles = (pl.ncol + 15) / 16;
             pl.ctiles = 1;
             const uint32_t tiles = pl.wtiles + pl.ctiles;
-            const uint64_t nks = Rpad / kKStep;
+            const uint64_t nks = Rpad / (int8 ? kI8KStep : kKStep);
             const uint64_t nblk = (Rpad + kFlushRows - 1) / kFlushRows;
             int *exps;
-            uint16_t *frag;
+            uint16_t *frag;   // 16 bytes per (k-step, tile, lane) in both layouts
             double *scales;
             if ((s = ws_typed(ws, 8, nblk * K, st, &exps)) != GNM_OK) return s;
             if ((s = ws_typed(ws, 9, nks * tiles * 64 * 8, st, &frag)) != GNM_OK) return s;
             if ((s = ws_typed(ws, 10, nblk * pl.ncol, st, &scales)) != GNM_OK) return s;
-            GNM_HIP(launch_block_exponent(d_weights, V, K, K, kFlushRows, L, exps, st));
-            GNM_HIP(launch_forward_frag(d_weights, V, K, L, exps, pl.wtiles, tiles, nks, frag, scales, st));
-            pl.bfrag = (const uint32_t *)frag;
+            GNM_HIP(launch_block_exponent(d_weights, V, K, K, kFlushRows, L, int8, exps, st));
             pl.scales = scales;
-            GNM_HIP(launch_score_mfma(pl, d_rows, bytes_per_variant, (N + 3) / 4, 0, 0, V, N, S, 0, tiles, true,
-                                      part, cpart, st));
+            pl.bfrag = (const uint32_t *)frag;
+            if (int8) {
+                uint8_t *flipbyte;
+                if ((s = ws_typed(ws, 14, Rpad, st, &flipbyte)) != GNM_OK) return s;
+                GNM_HIP(launch_flip_bytes(d_flip, V, Rpad, flipbyte, st));
+                GNM_HIP(launch_forward_frag_i8(d_weights, V, K, L, exps, pl.wtiles, tiles, nks, (uint32_t *)frag,
+                                               scales, st));
+                pl.int8 = 1;
+                pl.flipbyte = flipbyte;
+                GNM_HIP(launch_score_i8(pl, d_rows, bytes_per_variant, (N + 3) / 4, 0, 0, V, N, S, 0, tiles, true,
+                                        part, cpart, st));
+            } else {
+                GNM_HIP(launch_forward_frag(d_weights, V, K, L, exps, pl.wtiles, tiles, nks, frag, scales, st));
+                GNM_HIP(launch_score_mfma(pl, d_rows, bytes_per_variant, (N + 3) / 4, 0, 0, V, N, S, 0, tiles,
+                                          true, part, cpart, st));
+            }
         }
         GNM_HIP(launch_reduce(part, cpart, S, N, K, L, 1, score_class, d_scores, cnt_tmp, st));
         GNM_HIP(launch_take_col0(cnt_tmp, N, K, d_missing_per_person, st));
@@ -927,7 +1019,7 @@ gnm_status gnm_dense_adjoint_device(const uint8_t *d_rows, uint64_t bytes_per_va
         if (V && (!d_rows || !d_grad_weights)) return fail(GNM_ERR_INVALID_ARG, "NULL device pointer");
         if (!d_grad_scores) return fail(GNM_ERR_INVALID_ARG, "NULL grad_scores");
         if (bytes_per_variant < (N + 3) / 4) return fail(GNM_ERR_INVALID_ARG, "bytes_per_variant too small");
-        if (precision == GNM_PRECISION_AUTO) precision = GNM_PRECISION_BF16X3;
+        if (precision == GNM_PRECISION_AUTO || is_int8(precision)) precision = GNM_PRECISION_BF16X3;
         if (precision != GNM_PRECISION_BF16X2 && precision != GNM_PRECISION_BF16X3)
             return fail(GNM_ERR_UNSUPPORTED, "the adjoint runs in split-BF16 (bf16x2 or bf16x3)");
         const uint32_t L = precision == GNM_PRECISION_BF16X2 ? 2 : 3;
@@ -949,7 +1041,7 @@ gnm_status gnm_dense_adjoint_device(const uint8_t *d_rows, uint64_t bytes_per_va
         if ((s = ws_typed(ws, 12, nks * tiles * 64 * 8, st, &frag)) != GNM_OK) return s;
         if ((s = ws_typed(ws, 13, nblk * L * Kp, st, &scales)) != GNM_OK) return s;
         GNM_HIP(launch_fill_f64(scales, nblk * L * Kp, 0.0, st));
-        GNM_HIP(launch_block_exponent(d_grad_scores, N, K, K, kFlushRows, L, exps, st));
+        GNM_HIP(launch_block_exponent(d_grad_scores, N, K, K, kFlushRows, L, false, exps, st));
         GNM_HIP(launch_adjoint_frag(d_grad_scores, N, K, Kp, L, exps, tiles, nks, frag, scales, st));
         GNM_HIP(launch_adjoint(d_rows, bytes_per_variant, V, N, d_flip, (const uint32_t *)frag, scales, tiles, K, Kp,
                                L, d_grad_weights, st));

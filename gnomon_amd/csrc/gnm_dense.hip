@@ -46,10 +46,10 @@ __device__ __forceinline__ uint16_t limb_bf16(float w, int e, uint32_t L, uint32
 
 // --------------------------------------------------------------------------
 // exponent of the limb unit per (block of `block` rows, column):
-// smallest e with max|src| <= (2^(8L) - 2^(8(L-1))) * 2^e
+// smallest e with max|src| <= qmax * 2^e
 // --------------------------------------------------------------------------
 __global__ void k_block_exponent(const float *__restrict__ src, uint64_t rows, uint64_t ld,
-                                 uint64_t cols, uint64_t block, uint32_t L, int *__restrict__ out) {
+                                 uint64_t cols, uint64_t block, uint32_t L, int int8, int *__restrict__ out) {
     const uint64_t k = (uint64_t)blockIdx.x;
     const uint64_t blk = blockIdx.y;
     const uint64_t r0 = blk * block;
@@ -64,7 +64,7 @@ __global__ void k_block_exponent(const float *__restrict__ src, uint64_t rows, u
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        const double qmax = ldexp(1.0, 8 * L) - ldexp(1.0, 8 * (L - 1));
+        const double qmax = int8 ? 127.0 * ldexp(1.0, 8 * (L - 1)) : ldexp(1.0, 8 * L) - ldexp(1.0, 8 * (L - 1));
         const double mm = red[0];
         int e = 0;
         if (mm > 0.0) {
@@ -77,11 +77,11 @@ __global__ void k_block_exponent(const float *__restrict__ src, uint64_t rows, u
 }
 
 hipError_t launch_block_exponent(const float *src, uint64_t rows, uint64_t ld, uint64_t cols,
-                                 uint64_t block, uint32_t L, int *out, hipStream_t stream) {
+                                 uint64_t block, uint32_t L, bool int8, int *out, hipStream_t stream) {
     const uint64_t nblk = (rows + block - 1) / block;
     if (nblk == 0 || cols == 0) return hipSuccess;
     hipLaunchKernelGGL(k_block_exponent, dim3((unsigned)cols, (unsigned)nblk), dim3(256), 0, stream, src,
-                       rows, ld, cols, block, L, out);
+                       rows, ld, cols, block, L, int8 ? 1 : 0, out);
     return hipGetLastError();
 }
 

@@ -6,8 +6,10 @@
 
 namespace gnm {
 
+// int8: limb digits all in [-128, 127] (max |Q| = 127 * 256^(L-1)); else
+// BF16 limbs (max |Q| = 2^(8L) - 2^(8(L-1))).
 hipError_t launch_block_exponent(const float *src, uint64_t rows, uint64_t ld, uint64_t cols,
-                                 uint64_t block, uint32_t L, int *out, hipStream_t stream);
+                                 uint64_t block, uint32_t L, bool int8, int *out, hipStream_t stream);
 hipError_t launch_forward_frag(const float *W, uint64_t V, uint64_t K, uint32_t L, const int *exps,
                                uint32_t wtiles, uint32_t tiles, uint64_t nks, uint16_t *frag,
                                double *scales, hipStream_t stream);

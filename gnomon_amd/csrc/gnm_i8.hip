@@ -1,0 +1,485 @@
+// Split-int8 scoring kernel: the fused decode + contraction of
+// gnm_kernels.hip with weights written as signed 8-bit integer limbs and
+// v_mfma_i32_16x16x64_i8 (64 variants per MFMA, exact int32 accumulation).
+//
+// Per (scale block of 8192 program rows, weight column) the limb unit is a
+// power of two 2^e with max|w| <= 127 * 256^(L-1) * 2^e; w / 2^e is rounded
+// to an integer Q and written in L signed base-256 digits, every digit in
+// [-128, 127].  Dosages are 0/1/2, so each product is an exact small integer
+// and an int32 block sum is exact (8192 rows * 2 * 128 < 2^31); blocks are
+// flushed to FP64 with their scale.  L = 3 represents 23 bits, L = 4 31 bits
+// of the block maximum.
+//
+// Data path, per workgroup of WAVES waves (64 people per wave, 16*WAVES
+// packed bytes of every row) and per stage of 64 program rows (one k-step):
+//   * each thread loads 4 rows x 4 packed bytes (one dword per row;
+//     consecutive threads read consecutive dwords of a row), transposes the
+//     4x4 bytes in registers and writes 4 dwords into a column-major LDS tile
+//     [byte column][64 rows] (column pitch 80 bytes, row blocks swizzled per
+//     column so writes and 128-bit reads avoid bank conflicts);
+//   * one extra tile column holds a per-row selector byte: 4 for rows that
+//     read the A1 allele (flipped), 0 otherwise, and 0 for rows outside the
+//     split (whose data bytes are zero);
+//   * lane (i, g) of wave w reads column 16w + i, rows 16g..16g+15 as one
+//     128-bit LDS read: the packed codes of people 4(16w+i)+t, t = 0..3, for
+//     16 variants.  Person t's codes are isolated per byte and OR-ed with the
+//     row selector byte; one v_perm_b32 then maps (flip, code) to the i8
+//     dosage, another to the missing indicator:
+//        code 00 01 10 11  ->  dose 0 0 1 2 (A2 row) or 2 0 1 0 (A1 row),
+//                               missing 0 1 0 0;
+//   * M-tile t of the wave holds people 64w + 4i + t; B fragments (16 i8 per
+//     lane, the MFMA B operand) are staged through LDS per stage.
+// The register ring of D stages of unconditional loads and the LDS-only
+// stage barrier follow the split-BF16 kernel.
+#include "gnm_internal.hpp"
+
+#include <utility>
+
+namespace gnm {
+
+typedef int i32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int kI8Stage = 64;      // rows per stage = one k-step of v_mfma_i32_16x16x64_i8
+constexpr int kColPitch = 80;     // LDS bytes per tile column (64 rows + 16 pad)
+
+// LDS dword of (tile column col, row quad m): the four 16-row blocks of a
+// column are permuted by an XOR that depends on col / 16, so the transposed
+// dword writes of a wave (16 columns x 4 row quads) fall on distinct banks,
+// while the 128-bit reads of a 16-row block stay contiguous and aligned.
+__device__ __forceinline__ int tile_dword(int col, int m) {
+    return col * (kColPitch / 4) + (m ^ (4 * ((col >> 4) & 3)));
+}
+
+__device__ __forceinline__ void lds_barrier_i8() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// 4x4 byte transpose: r[k] byte b -> c[b] byte k.
+__device__ __forceinline__ void transpose4x4(const uint32_t (&r)[4], uint32_t (&c)[4]) {
+    const uint32_t lo01 = __builtin_amdgcn_perm(r[1], r[0], 0x05010400u);
+    const uint32_t hi01 = __builtin_amdgcn_perm(r[1], r[0], 0x07030602u);
+    const uint32_t lo23 = __builtin_amdgcn_perm(r[3], r[2], 0x05010400u);
+    const uint32_t hi23 = __builtin_amdgcn_perm(r[3], r[2], 0x07030602u);
+    c[0] = __builtin_amdgcn_perm(lo23, lo01, 0x05040100u);
+    c[1] = __builtin_amdgcn_perm(lo23, lo01, 0x07060302u);
+    c[2] = __builtin_amdgcn_perm(hi23, hi01, 0x05040100u);
+    c[3] = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
+}
+
+// selector bytes (code | flip<<2) -> i8 dosage / missing indicator
+__device__ __forceinline__ uint32_t dose_i8(uint32_t sel) {
+    return __builtin_amdgcn_perm(0x00010002u, 0x02010000u, sel);
+}
+__device__ __forceinline__ uint32_t miss_i8(uint32_t sel) {
+    return __builtin_amdgcn_perm(0x00000100u, 0x00000100u, sel);
+}
+
+}  // namespace
+
+template <int TMAX, int WAVES, int D, bool VEC, bool IDENT>
+__global__ __launch_bounds__(64 * WAVES) void k_score_i8(DevicePlan plan, const uint8_t *__restrict__ rows,
+                                                         uint64_t stride, uint64_t bpv, uint64_t v0, uint64_t p0,
+                                                         uint64_t p1, uint64_t split_base, uint64_t rps,
+                                                         uint32_t splits, uint32_t person_tiles,
+                                                         uint64_t num_people, uint32_t tile_begin,
+                                                         uint32_t ntile, double *__restrict__ part,
+                                                         uint32_t *__restrict__ cpart) {
+    constexpr int kBytes = 16 * WAVES;               // packed bytes per row per workgroup
+    constexpr int kCols = kBytes + 1;                // + the selector column
+    constexpr int kTileWords = kCols * kColPitch / 4;
+    constexpr int kNB = (TMAX + WAVES - 1) / WAVES;  // B chunks per thread per stage (some idle when TMAX < WAVES)
+    static_assert(kFlushRows % (D * kI8Stage) == 0, "scale blocks must hold whole ring turns");
+    __shared__ uint32_t tile[2][kTileWords];
+    __shared__ u32x4_t btile[2][TMAX * 64];
+    __shared__ int stagebuf[WAVES][64 * 16];
+
+    // XCD-aware placement, as in the split-BF16 kernel
+    const uint32_t L = blockIdx.x;
+    const uint32_t xcd = L & 7u, slot = L >> 3;
+    const uint32_t s = xcd + 8u * (slot / person_tiles);
+    const uint32_t ptile = slot % person_tiles;
+    if (s >= splits) return;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    const uint64_t col0 = (uint64_t)ptile * kBytes;
+    const uint64_t pw0 = (uint64_t)ptile * (64 * WAVES) + (uint64_t)wave * 64;
+    const uint64_t sb = split_base + (uint64_t)s * rps;
+    const uint64_t r_begin = p0 > sb ? p0 : sb;
+    const uint64_t se = sb + rps;
+    const uint64_t r_end = p1 < se ? p1 : se;
+    const uint32_t wt = plan.wtiles;
+    const uint32_t tiles_total = plan.wtiles + plan.ctiles;
+    const uint32_t nw = tile_begin < wt ? ((wt - tile_begin) < ntile ? (wt - tile_begin) : ntile) : 0;
+    const bool need_miss = nw < ntile;
+    // load role: rows 4rg..4rg+3, packed bytes 4c4..4c4+3 of the workgroup's span
+    // (consecutive threads take consecutive dwords of a row: each load
+    // instruction reads 64 contiguous bytes of each of its rows)
+    const int rg = tid / (4 * WAVES), c4 = tid % (4 * WAVES);
+    const uint64_t ld_col = col0 + 4 * (uint64_t)c4;
+    const bool ld_any = ld_col < bpv;
+    // Every stage issues the same loads on every path (one dword per row):
+    // past the row end the dword is read at the last full dword of the row
+    // (bpv >= 4 on the vector path) and shifted / masked back at store time.
+    const uint64_t ld_colc = ld_col + 4 <= bpv ? ld_col : bpv - 4;
+    const uint32_t col_shift = ld_any ? (uint32_t)(ld_col - ld_colc) * 8u : 0u;
+    const uint32_t col_mask = ld_col + 4 <= bpv ? 0xFFFFFFFFu
+                              : (ld_any ? (1u << (8u * (uint32_t)(bpv - ld_col))) - 1u : 0u);
+    const u32x4_t *__restrict__ bfrag = (const u32x4_t *)plan.bfrag;
+    const uint32_t *__restrict__ fbw = (const uint32_t *)plan.flipbyte;
+
+    uint32_t ra[D][4];      // row dwords of each stage in flight
+    u32x4_t rB[D][kNB];     // B chunks
+    uint32_t rs[D];         // selector bytes of rows 4rg..4rg+3 (threads with c4 == 0)
+    uint32_t rk[D];         // keep mask bits 0..3 for rows 4rg+k inside the split
+    bool rfull[D];          // whole stage inside the split (uniform): no masking
+    // Fast path: every row of the stage is inside [r_begin, r_end) of an
+    // identity block and the workgroup's byte span lies inside the row, so
+    // the four row loads are one per-thread pointer plus uniform offsets.
+    const bool span_full = VEC && col0 + kBytes <= bpv;
+    const uint8_t *bed_row0 = rows + (uint64_t)(4 * rg) * stride + (span_full ? ld_col : 0);
+
+    const uint64_t last_st = ((r_end - 1) / kI8Stage) * kI8Stage;
+    uint32_t boff[kNB];
+#pragma unroll
+    for (int m = 0; m < kNB; ++m) {
+        const int q = tid + m * 64 * WAVES;
+        const int j = q / 64, l = q % 64;
+        // tiles >= ntile (and chunks past TMAX tiles) are loaded clamped and never read
+        const uint32_t jc = (uint32_t)j < ntile ? (uint32_t)j : ntile - 1;
+        boff[m] = (uint32_t)((tile_begin + jc) * 64 + l);
+    }
+    auto load_row = [&](uint64_t var) __attribute__((always_inline)) -> uint32_t {
+        const uint8_t *src = rows + var * stride;
+        if (VEC) return *(const uint32_t *)(src + ld_colc);
+        uint32_t x = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const uint64_t o = ld_col + b;
+            const uint32_t byte = src[o < bpv ? o : bpv - 1];
+            x |= (o < bpv ? byte : 0u) << (8 * b);
+        }
+        return x;
+    };
+    auto issue = [&](int d, uint64_t st_req) __attribute__((always_inline)) {
+        if (IDENT && span_full && st_req >= r_begin && st_req + kI8Stage <= r_end) {
+            const uint8_t *p = bed_row0 + (st_req - p0) * stride;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) ra[d][k] = *(const uint32_t *)(p + (uint64_t)k * stride);
+            rk[d] = 0xFu;
+            rfull[d] = true;
+            rs[d] = fbw[(st_req >> 2) + rg];
+            const u32x4_t *bs = bfrag + (st_req / kI8Stage) * tiles_total * 64;
+#pragma unroll
+            for (int m = 0; m < kNB; ++m) rB[d][m] = bs[boff[m]];
+            return;
+        }
+        const uint64_t st = st_req < last_st ? st_req : last_st;
+        const bool same = st == st_req;
+        uint32_t keep = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint64_t r = st + 4 * rg + k;
+            const bool ok = same && r >= r_begin && r < r_end && ld_any;
+            keep |= ok ? (1u << k) : 0u;
+            const uint64_t rc = r < r_begin ? r_begin : (r >= r_end ? r_end - 1 : r);
+            const uint64_t var = IDENT ? (rc - p0) : (uint64_t)(plan.prog_var[rc] - (uint32_t)v0);
+            ra[d][k] = load_row(var);
+        }
+        rk[d] = keep;
+        rfull[d] = false;
+        rs[d] = fbw[(st >> 2) + rg];   // selector bytes of rows st+4rg..+3 (used by c4 == 0)
+        const u32x4_t *bs = bfrag + (st / kI8Stage) * tiles_total * 64;
+#pragma unroll
+        for (int m = 0; m < kNB; ++m) rB[d][m] = bs[boff[m]];
+    };
+    auto store = [&](int buf, int d) __attribute__((always_inline)) {
+        uint32_t r[4], c[4];
+        const uint32_t keep = rk[d];
+        if (rfull[d]) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) r[k] = ra[d][k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t x = VEC ? (ra[d][k] >> col_shift) & col_mask : ra[d][k];
+                r[k] = ((keep >> k) & 1u) ? x : 0u;
+            }
+        }
+        transpose4x4(r, c);
+        uint32_t *t = tile[buf];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) t[tile_dword(4 * c4 + b, rg)] = c[b];
+        if (c4 == 0) {
+            // selector bytes, zero for rows outside the split
+            uint32_t sel = rs[d];
+            if (!rfull[d]) {
+                const uint32_t km = ((keep & 1u) ? 0xFFu : 0u) | ((keep & 2u) ? 0xFF00u : 0u) |
+                                    ((keep & 4u) ? 0xFF0000u : 0u) | ((keep & 8u) ? 0xFF000000u : 0u);
+                sel &= km;
+            }
+            t[tile_dword(kBytes, rg)] = sel;
+        }
+#pragma unroll
+        for (int m = 0; m < kNB; ++m)
+            if (kNB * 64 * WAVES == TMAX * 64 || tid + m * 64 * WAVES < TMAX * 64)
+                btile[buf][tid + m * 64 * WAVES] = rB[d][m];
+    };
+
+    i32x4_t acc[4][TMAX];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int j = 0; j < TMAX; ++j) acc[t][j] = i32x4_t{0, 0, 0, 0};
+
+    auto flush = [&](uint64_t blk, bool first, bool empty) __attribute__((always_inline)) {
+        auto flush_tile = [&](auto jc) __attribute__((always_inline)) {
+            constexpr int j = decltype(jc)::value;
+            if ((uint32_t)j >= ntile) return;
+            lds_barrier_i8();
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) stagebuf[wave][(16 * lg + 4 * q + t) * 16 + li] = acc[t][j][q];
+                acc[t][j] = i32x4_t{0, 0, 0, 0};
+            }
+            lds_barrier_i8();
+            const uint32_t tl = tile_begin + j;
+            const bool wtile = tl < wt;
+            const uint32_t c = wtile ? tl * 16 + li : (tl - wt) * 16 + li;
+            const uint32_t ncols = wtile ? plan.ncol : plan.nclass;
+            if (c < ncols) {
+                const double sc = (wtile && !empty) ? plan.scales[blk * plan.ncol + c] : 0.0;
+#pragma unroll 1
+                for (int i = 0; i < 16; ++i) {
+                    const uint32_t pl = 4 * i + lg;  // person of the wave, 0..63
+                    const uint64_t p = pw0 + pl;
+                    if (p >= num_people) continue;
+                    const int v = stagebuf[wave][pl * 16 + li];
+                    const uint64_t row = (uint64_t)s * num_people + p;
+                    if (wtile) {
+                        double *dd = part + row * plan.ncol + c;
+                        const double x = (double)v * sc;
+                        *dd = first ? x : *dd + x;
+                    } else {
+                        uint32_t *dd = cpart + row * plan.nclass + c;
+                        const uint32_t x = (uint32_t)v;
+                        *dd = first ? x : *dd + x;
+                    }
+                }
+            }
+        };
+        [&]<int... J>(std::integer_sequence<int, J...>) __attribute__((always_inline)) {
+            (flush_tile(std::integral_constant<int, J>{}), ...);
+        }(std::make_integer_sequence<int, TMAX>{});
+    };
+
+    auto compute = [&](int buf) __attribute__((always_inline)) {
+        const uint32_t *t = tile[buf];
+        const u32x4_t X = *(const u32x4_t *)&t[tile_dword(wave * 16 + li, 4 * lg)];
+        const u32x4_t F = *(const u32x4_t *)&t[tile_dword(kBytes, 4 * lg)];
+        uint32_t sel[4][4];   // [person t][dword q]
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int tt = 0; tt < 4; ++tt) sel[tt][q] = ((X[q] >> (2 * tt)) & 0x03030303u) | F[q];
+        i32x4_t A[4];
+#pragma unroll
+        for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) A[tt][q] = (int)dose_i8(sel[tt][q]);
+#pragma unroll
+        for (int j = 0; j < TMAX; ++j) {
+            if ((uint32_t)j < nw) {
+                const u32x4_t bv = btile[buf][j * 64 + lane];
+                const i32x4_t B = i32x4_t{(int)bv[0], (int)bv[1], (int)bv[2], (int)bv[3]};
+#pragma unroll
+                for (int tt = 0; tt < 4; ++tt)
+                    acc[tt][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[tt], B, acc[tt][j], 0, 0, 0);
+            }
+        }
+        if (need_miss) {
+            i32x4_t M[4];
+#pragma unroll
+            for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) M[tt][q] = (int)miss_i8(sel[tt][q]);
+#pragma unroll
+            for (int j = 0; j < TMAX; ++j) {
+                if ((uint32_t)j >= nw && (uint32_t)j < ntile) {
+                    const u32x4_t bv = btile[buf][j * 64 + lane];
+                    const i32x4_t B = i32x4_t{(int)bv[0], (int)bv[1], (int)bv[2], (int)bv[3]};
+#pragma unroll
+                    for (int tt = 0; tt < 4; ++tt)
+                        acc[tt][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(M[tt], B, acc[tt][j], 0, 0, 0);
+                }
+            }
+        }
+    };
+
+    if (r_begin >= r_end) {
+        flush(0, true, true);
+        return;
+    }
+    // Stages are processed in batches of D: the loads of all D stages are
+    // issued together at the top of a batch and consumed stage by stage.  No
+    // load result is carried across loop iterations (a value loaded in one
+    // iteration and consumed in the next gets copied by the register
+    // allocator, and the copy waits for every load in flight).
+    const uint64_t st0 = (r_begin / (D * kI8Stage)) * (D * kI8Stage);
+    const uint64_t nst = (r_end - st0 + kI8Stage - 1) / kI8Stage;
+    bool first = true;
+    for (uint64_t i0 = 0; i0 < nst; i0 += D) {
+        [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
+            (issue(I, st0 + (i0 + I) * kI8Stage), ...);
+        }(std::make_integer_sequence<int, D>{});
+        auto step = [&](auto dc) __attribute__((always_inline)) {
+            constexpr int d = decltype(dc)::value;
+            if (i0 + d >= nst) return;
+            const int buf = (int)((i0 + d) & 1);
+            store(buf, d);
+            lds_barrier_i8();
+            compute(buf);
+        };
+        [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
+            (step(std::integral_constant<int, I>{}), ...);
+        }(std::make_integer_sequence<int, D>{});
+        const uint64_t st_next = st0 + (i0 + D) * kI8Stage;
+        if ((st_next % kFlushRows) == 0 || i0 + D >= nst) {
+            flush((st0 + i0 * kI8Stage) / kFlushRows, first, false);
+            first = false;
+        }
+    }
+}
+
+#ifndef GNM_I8_BATCH
+#define GNM_I8_BATCH 4   // stages whose loads are issued together
+#endif
+#ifndef GNM_I8_WAVES
+#define GNM_I8_WAVES 4   // waves per workgroup (64 people each)
+#endif
+
+hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t stride, uint64_t bpv, uint64_t v0,
+                           uint64_t p0, uint64_t p1, uint64_t num_people, uint32_t splits, uint32_t tile_begin,
+                           uint32_t tile_end, bool identity, double *part, uint32_t *cpart, hipStream_t stream) {
+    const bool vec = ((reinterpret_cast<uintptr_t>(rows) | stride) & 3u) == 0 && bpv >= 4;
+    const uint64_t split_base = (p0 / kI8Stage) * kI8Stage;
+    uint64_t rows_per_split = (p1 - split_base + splits - 1) / splits;
+    rows_per_split = (rows_per_split + kI8Stage - 1) / kI8Stage * kI8Stage;
+    for (uint32_t tb = tile_begin; tb < tile_end; tb += kMfmaMaxTiles) {
+        const uint32_t nt = (tile_end - tb) < (uint32_t)kMfmaMaxTiles ? (tile_end - tb) : (uint32_t)kMfmaMaxTiles;
+#define GNM_LAUNCH_I8(TM, WV, DD, VC, ID)                                                                      \
+    do {                                                                                                       \
+        const uint32_t pt = (uint32_t)((num_people + 64 * WV - 1) / (64 * WV));                                \
+        const uint64_t wgs = (uint64_t)pt * splits;                                                            \
+        hipLaunchKernelGGL((k_score_i8<TM, WV, DD, VC, ID>), dim3((unsigned)wgs), dim3(64 * WV), 0, stream,      \
+                           plan, rows, stride, bpv, v0, p0, p1, split_base, rows_per_split, splits, pt,        \
+                           num_people, tb, nt, part, cpart);                                                   \
+    } while (0)
+#define GNM_LAUNCH_I8_V(TM, WV, DD)                                                                            \
+    do {                                                                                                       \
+        if (vec) {                                                                                             \
+            if (identity) GNM_LAUNCH_I8(TM, WV, DD, true, true); else GNM_LAUNCH_I8(TM, WV, DD, true, false);   \
+        } else {                                                                                               \
+            if (identity) GNM_LAUNCH_I8(TM, WV, DD, false, true); else GNM_LAUNCH_I8(TM, WV, DD, false, false); \
+        }                                                                                                      \
+    } while (0)
+        if (nt <= 4) GNM_LAUNCH_I8_V(4, GNM_I8_WAVES, GNM_I8_BATCH);
+        else GNM_LAUNCH_I8_V(kMfmaMaxTiles, GNM_I8_WAVES, GNM_I8_BATCH);
+#undef GNM_LAUNCH_I8_V
+#undef GNM_LAUNCH_I8
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// Device-side builders of the dense (differentiable) path.
+// ---------------------------------------------------------------------------
+
+// signed base-256 digit j (of L) of the integer nearest w / 2^e, all digits in [-128, 127]
+__device__ __forceinline__ int32_t limb_i8(float w, int e, uint32_t L, uint32_t j) {
+    if (w == 0.0f) return 0;
+    int64_t Q = (int64_t)rint(ldexp((double)w, -e));
+    for (uint32_t i = L - 1; i > j; --i) {
+        const int64_t d = ((Q + 128) & 255) - 128;
+        Q = (Q - d) / 256;
+    }
+    return j == 0 ? (int32_t)Q : (int32_t)(((Q + 128) & 255) - 128);
+}
+
+// B fragments [k-step of 64 rows][tile][lane][16 i8] from dense W [V][K]:
+// lane (n = lane%16, g = lane/16) byte b holds row 64ks + 16g + b of column
+// tile*16 + n; column c = limb*K + k; the count tile (class 0) is 1 on every
+// real row.  One thread per (k-step, tile, lane).
+__global__ void k_forward_frag_i8(const float *__restrict__ W, uint64_t V, uint64_t K, uint32_t L,
+                                  const int *__restrict__ exps, uint32_t wtiles, uint32_t tiles, uint64_t nks,
+                                  uint32_t *__restrict__ frag, double *__restrict__ scales) {
+    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= nks * tiles * 64) return;
+    const uint32_t lane = id & 63;
+    const uint32_t tile = (uint32_t)((id >> 6) % tiles);
+    const uint64_t ks = (id >> 6) / tiles;
+    const uint32_t n = lane & 15, g = lane >> 4;
+    const uint64_t ncol = (uint64_t)L * K;
+    const uint64_t blk = ks * kI8Stage / kFlushRows;
+    uint32_t out[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+        const uint64_t r = ks * kI8Stage + g * 16 + b;
+        int32_t val = 0;
+        if (r < V) {
+            if (tile < wtiles) {
+                const uint64_t c = (uint64_t)tile * 16 + n;
+                if (c < ncol) {
+                    const uint32_t limb = (uint32_t)(c / K);
+                    const uint64_t k = c % K;
+                    val = limb_i8(W[r * K + k], exps[blk * K + k], L, limb);
+                }
+            } else if (tile == wtiles && n == 0) {
+                val = 1;
+            }
+        }
+        out[b >> 2] |= ((uint32_t)val & 0xFFu) << (8 * (b & 3));
+    }
+    uint32_t *dst = frag + id * 4;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dst[q] = out[q];
+    if (tile < wtiles && g == 0 && (ks * kI8Stage) % kFlushRows == 0) {
+        const uint64_t c = (uint64_t)tile * 16 + n;
+        if (c < ncol) {
+            const uint32_t limb = (uint32_t)(c / K);
+            const uint64_t k = c % K;
+            scales[blk * ncol + c] = ldexp(1.0, exps[blk * K + k] + 8 * (int)(L - 1 - limb));
+        }
+    }
+}
+
+hipError_t launch_forward_frag_i8(const float *W, uint64_t V, uint64_t K, uint32_t L, const int *exps,
+                                  uint32_t wtiles, uint32_t tiles, uint64_t nks, uint32_t *frag, double *scales,
+                                  hipStream_t stream) {
+    const uint64_t n = nks * tiles * 64;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_forward_frag_i8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, W, V, K, L, exps,
+                       wtiles, tiles, nks, frag, scales);
+    return hipGetLastError();
+}
+
+// per-row selector bytes (4 = flipped row) from a flip byte array
+__global__ void k_flip_bytes(const uint8_t *__restrict__ flip, uint64_t V, uint64_t Rpad, uint8_t *__restrict__ out) {
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < Rpad) out[r] = (flip && r < V && flip[r]) ? 4u : 0u;
+}
+
+hipError_t launch_flip_bytes(const uint8_t *flip, uint64_t V, uint64_t Rpad, uint8_t *out, hipStream_t stream) {
+    if (Rpad == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_flip_bytes, dim3((unsigned)((Rpad + 255) / 256)), dim3(256), 0, stream, flip, V, Rpad, out);
+    return hipGetLastError();
+}
+
+}  // namespace gnm

@@ -31,7 +31,9 @@ struct DevicePlan {
     // split-BF16 path
     const uint32_t *bfrag;     // [R/32][tiles][64 lanes][4 dwords] B fragments (weights then counts)
     const double *scales;      // [R/kFlushRows][ncol] power-of-two limb scales
-    uint32_t limbs;            // 2 or 3
+    uint32_t limbs;            // 2 or 3 (BF16 limbs), 3 or 4 (int8 limbs)
+    uint32_t int8;             // 1: int8 limbs, fragments [R/64][tiles][64 lanes][16 i8]
+    const uint8_t *flipbyte;   // [Rpad] int8 path: 4 for rows that read the A1 allele, else 0
     uint32_t ncol;             // limbs * K (weight columns, column = limb*K + score)
     uint32_t wtiles;           // ceil(ncol/16)
     uint32_t ctiles;           // ceil(nclass/16)
@@ -64,6 +66,16 @@ hipError_t launch_score_mfma(const DevicePlan &plan, const uint8_t *rows, uint64
                              uint64_t num_people, uint32_t splits,
                              uint32_t tile_begin, uint32_t tile_end, bool identity, double *part,
                              uint32_t *cpart, hipStream_t stream);
+// Split-int8 scoring (gnm_i8.hip); rows must be 4-byte aligned with a
+// 4-byte multiple stride for the dword staging loads (else byte loads).
+hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t stride, uint64_t bpv, uint64_t v0,
+                           uint64_t p0, uint64_t p1, uint64_t num_people, uint32_t splits, uint32_t tile_begin,
+                           uint32_t tile_end, bool identity, double *part, uint32_t *cpart, hipStream_t stream);
+hipError_t launch_forward_frag_i8(const float *W, uint64_t V, uint64_t K, uint32_t L, const int *exps,
+                                  uint32_t wtiles, uint32_t tiles, uint64_t nks, uint32_t *frag, double *scales,
+                                  hipStream_t stream);
+hipError_t launch_flip_bytes(const uint8_t *flip, uint64_t V, uint64_t Rpad, uint8_t *out, hipStream_t stream);
+constexpr int kI8KStep = 64;           // rows per k-step of the int8 kernel
 // scores[p][k] += sum_s sum_l part[s][p][l*K + k]; counts[p][k] += sum_s cpart[s][p][class[k]]
 hipError_t launch_reduce(const double *part, const uint32_t *cpart, uint32_t splits,
                          uint64_t num_people, uint64_t num_scores, uint32_t limbs,

@@ -54,12 +54,20 @@ typedef enum gnm_status {
  *           product and every FP32 partial sum inside a variant block is exact;
  *           blocks are combined in FP64.  Replaces the split-FP16 tensor-core
  *           path (cuda_backend.rs:2360-2415).
+ *   I8X3/I8X4: split-int8 MFMA (v_mfma_i32_16x16x64_i8).  Each weight column
+ *           is written as 3 or 4 signed 8-bit integer limbs times a power-of-two
+ *           scale per (variant block, column); products and int32 block sums
+ *           are exact; blocks are combined in FP64.  I8X4 represents 31 bits
+ *           of each block maximum, I8X3 23 bits.  Same role as the split paths
+ *           above (no reference counterpart for the int8 limbs themselves).
  *   AUTO:   BF16X3. */
 typedef enum gnm_precision {
     GNM_PRECISION_FP32 = 0,
     GNM_PRECISION_BF16X2 = 1,
     GNM_PRECISION_BF16X3 = 2,
-    GNM_PRECISION_AUTO = 3
+    GNM_PRECISION_AUTO = 3,
+    GNM_PRECISION_I8X3 = 4,
+    GNM_PRECISION_I8X4 = 5
 } gnm_precision;
 
 /* The slice of PreparationResult (score/types.rs:166-365) the hot path reads. */

@@ -4,8 +4,9 @@ and the FP64 CPU gradient.
 
 Error bounds follow from the split-limb construction: a value x in a
 (block, column) cell with maximum magnitude m is represented as an integer
-multiple of a power-of-two unit u with m <= (2^(8L) - 2^(8(L-1))) u, so
-|x - x_hat| <= u / 2 <= m / (2^(8L+1) - 2^(8L-7)).  Dosages are exact, and
+multiple of the smallest power of two u with m <= qmax u (qmax = 2^(8L) -
+2^(8(L-1)) for BF16 limbs, 127 * 256^(L-1) for int8 limbs), so u < 2 m / qmax
+and |x - x_hat| <= u / 2 < m / qmax.  Dosages are exact, and
 products and partial sums inside a block are exact (integers below 2^24), so
 the contraction error is the sum of those representation errors weighted by
 the dosages.
@@ -23,7 +24,12 @@ from helpers import pack_bed
 pytestmark = pytest.mark.gpu
 
 BLOCK = 8192   # rows (forward: variants; adjoint: people) per scale block
-LIMBS = {"bf16x3": 3, "bf16x2": 2}
+LIMBS = {"bf16x3": 3, "bf16x2": 2, "i8x4": 4, "i8x3": 3}
+
+
+def _qmax(precision):
+    L = LIMBS[precision]
+    return 127.0 * 256.0 ** (L - 1) if precision.startswith("i8") else 2.0 ** (8 * L) - 2.0 ** (8 * (L - 1))
 
 
 def _flush(msg):
@@ -47,10 +53,10 @@ def _effect_dosage(dos, flip):
     return g
 
 
-def _limb_bound(G, X, L, axis_rows):
+def _limb_bound(G, X, precision):
     """max error of sum over rows r of G[r, :] * X[r, k] when X is limb-split
-    per BLOCK rows and column: sum_blk (sum_{r in blk} G[r, i]) * m_blk,k / d."""
-    d = 2.0 ** (8 * L + 1) - 2.0 ** (8 * L - 7)
+    per BLOCK rows and column: sum_blk (sum_{r in blk} G[r, i]) * m_blk,k / qmax."""
+    d = _qmax(precision)
     R = X.shape[0]
     out = 0.0
     for b0 in range(0, R, BLOCK):
@@ -64,7 +70,7 @@ def _dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16x2"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16x2", "i8x4", "i8x3"])
 def test_dense_forward_matches_fp64(orc, precision):
     N, V, K = 3001, 9000, 16          # two variant scale blocks, ragged N and V
     _flush(f"dense forward {precision} N={N} V={V} K={K}")
@@ -79,9 +85,9 @@ def test_dense_forward_matches_fp64(orc, precision):
         assert err.max() <= 1e-12 * np.abs(S64).max()
     else:
         G = _effect_dosage(dos, flip)                 # [V, N]
-        bound = _limb_bound(G, W.astype(np.float64), LIMBS[precision], "variants")
+        bound = _limb_bound(G, W.astype(np.float64), precision)
         assert np.all(err <= bound + 1e-12 * np.abs(S64).max()), (err - bound).max()
-        if precision == "bf16x3":
+        if precision in ("bf16x3", "i8x4"):
             assert err.max() / np.abs(S64).max() <= 1e-6
 
 
@@ -98,7 +104,7 @@ def test_adjoint_matches_fp64(orc, precision, K):
     torch.cuda.synchronize()
     dW = dW.cpu().numpy().astype(np.float64)
     G = _effect_dosage(dos, flip).T                   # [N, V]
-    bound = _limb_bound(G, R.astype(np.float64), LIMBS[precision], "people")
+    bound = _limb_bound(G, R.astype(np.float64), precision)
     err = np.abs(dW - dW64)
     assert np.all(err <= bound + 2.0 ** -23 * np.abs(dW64) + 1e-30), (err - bound).max()
 

@@ -23,7 +23,7 @@ from tests.helpers import gpu_tests_genotype, normwise_rel, pack_bed, reference_
 pytestmark = pytest.mark.gpu
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))
 LD = os.path.join(os.path.dirname(__file__), "golden", "ld.bed")
-TOL = {"fp32": None, "bf16x3": 1e-6, "bf16x2": 1e-4}
+TOL = {"fp32": None, "bf16x3": 1e-6, "bf16x2": 1e-4, "i8x4": 1e-6, "i8x3": 1e-5}
 
 
 def _flush(msg):
@@ -36,13 +36,13 @@ def _dev(a):
 
 def _check_scores(S, C, S64, C64, precision):
     assert np.array_equal(C, C64), "missing counts differ"
-    if precision == "bf16x2":
-        assert normwise_rel(S, S64).max() <= TOL["bf16x2"]
+    if precision in ("bf16x2", "i8x3"):
+        assert normwise_rel(S, S64).max() <= TOL[precision]
         return
     ok = reference_agree(S, S64, 1e-6)
     assert ok.all(), f"{(~ok).sum()} cells outside the 1e-6 rule; worst {np.abs(S - S64).max()}"
-    if precision == "bf16x3":
-        assert normwise_rel(S, S64).max() <= TOL["bf16x3"]
+    if precision in ("bf16x3", "i8x4"):
+        assert normwise_rel(S, S64).max() <= TOL[precision]
 
 
 def _score(csr, rows, N, precision, nfam=None, o2f=None, blocks=1):
@@ -114,7 +114,7 @@ def test_unpack_ld_fixture(orc):
     assert np.array_equal(d_out.cpu().numpy().reshape(V, N), orc.decode_rows(rows, N))
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16x2"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16x2", "i8x4", "i8x3"])
 def test_plain_small_config(orc, precision):
     """2,000 people x 50,000 variants x 1 score, HWE p~U(0.05,0.5), 1% missing,
     w ~ N(0, 0.01^2), effect allele A1 for a random half."""
@@ -128,7 +128,7 @@ def test_plain_small_config(orc, precision):
     _check_scores(S, C, S64, C64, precision)
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "i8x4"])
 def test_many_scores_classes_and_mixed_flips(orc, precision):
     """K=70 with per-cell presence (membership classes > 4) and per-cell flips
     (variants needing both an A2 and an A1 row); N not a multiple of 4."""
@@ -152,7 +152,7 @@ def test_many_scores_classes_and_mixed_flips(orc, precision):
     assert reference_agree(S, Sr, 2e-6).all()
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "i8x4"])
 def test_person_subset_gather(orc, precision):
     _flush(f"subset {precision}")
     rng = np.random.default_rng(9)
@@ -173,7 +173,7 @@ def _single(w_eff, corr, K=1):
                     score_variant_counts=np.ones(K, np.uint32))
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "i8x4"])
 def test_reference_golden_vectors_on_device(precision):
     _flush(f"golden {precision}")
     g = GOLD["variant_major_missing_correction"]
@@ -199,7 +199,7 @@ def test_reference_golden_vectors_on_device(precision):
     assert abs(S[0, 0] - g["sum"]) < 1e-6 and C[0, 0] == 0
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "i8x4"])
 def test_split_self_test_known_answer(precision):
     """Inputs of the reference's split self-test (cuda_backend.rs:2432-2438):
     dosage[p][v] = (64p+v) % 3, weight[v][k] = ((7(64v+k)) mod 101 - 50)/13.
@@ -218,9 +218,12 @@ def test_split_self_test_known_answer(precision):
     if precision == "fp32":
         assert reference_agree(S, exact_f32w, 1e-6).all()
     else:
-        # fixed-point limbs: |w - Q u| <= u/2 <= max|w| / (2^24 - 2^16) per weight,
-        # so |dS| <= that bound times sum_v d (a hard bound, not a statistic)
-        bound = np.abs(W64).max() / (2.0**24 - 2.0**16) * dos.sum(1, keepdims=True)
+        # fixed-point limbs: the unit u = 2^e is the smallest power of two with
+        # max|w| <= qmax u, so u < 2 max|w| / qmax and |w - Q u| <= u/2 <
+        # max|w| / qmax per weight (qmax = 2^24 - 2^16 for three BF16 limbs,
+        # 127 * 256^3 for four int8 limbs); |dS| <= that times sum_v d
+        qmax = {"bf16x3": 2.0**24 - 2.0**16, "i8x4": 127.0 * 256.0**3}[precision]
+        bound = np.abs(W64).max() / qmax * dos.sum(1, keepdims=True)
         assert np.all(np.abs(S - exact_f32w) <= bound + 1e-12)
         assert normwise_rel(S, exact_f32w).max() <= 1e-6
     assert np.abs(S - exact).max() < 1e-4   # f32 weights vs the rational thirteenths
