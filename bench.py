@@ -7,10 +7,20 @@ missing), weights ~ N(0, 0.01^2), effect allele A1 for a random half.  The
 in HBM; one step is one pass of the scoring kernels over every variant,
 accumulated into the FP64 result.
 
+Headline path: split-int8 MFMA with four limbs (i8x4: weights as 4 signed
+8-bit limbs per power-of-two block scale, exact int32 block sums, FP64 across
+blocks); the plain FP32 path and the split-BF16 path (bf16x3) are timed on the
+same rows for comparison.
+
 Prints one JSON line: metric svs/s = people * variants * scores / second,
 plus the roofline of the dominant kernel (HIP-event time on the library's
 compute stream) and the CPU baseline (restated reference CPU engine on a
 bounded sample of the same workload, all host cores).
+
+Other modes: --workload precision_64 (per-score error report of every path
+against the FP64 oracle), --workload adjoint_16 (differentiable path: 20
+gradient steps, finite-difference and FP64 gradient checks),
+--calibrate-copy MIB (a plain copy of known size for the HBM counters).
 """
 from __future__ import annotations
 
@@ -312,8 +322,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="plain_fp32_biobank")
-    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16x2", "bf16x3", "i8x3", "i8x4"])
-    ap.add_argument("--compare", default="fp32", help="second path timed for the report ('' = none)")
+    ap.add_argument("--precision", default="i8x4", choices=["fp32", "bf16x2", "bf16x3", "i8x3", "i8x4"])
+    ap.add_argument("--compare", default="fp32,bf16x3",
+                    help="comma-separated further paths timed on the same rows for the report ('' = none)")
     ap.add_argument("--variants", type=int, default=0, help="override the variant count")
     ap.add_argument("--cpu-sample-variants", type=int, default=30000)
     ap.add_argument("--no-cpu", action="store_true")
@@ -402,12 +413,12 @@ def main():
         log(f"check: {check}")
     ctx.close()
 
-    compare = None
-    if args.compare and args.compare != args.precision:
-        c2, ms2, main2 = run_path(args.compare)
+    compare = []
+    for prec in [p for p in args.compare.split(",") if p and p != args.precision]:
+        c2, ms2, main2 = run_path(prec)
         c2.close()
-        compare = {"precision": args.compare, "ms_per_step": ms2, "value": svs / (ms2 / 1e3),
-                   "main_kernel_ms": main2, "speedup_of_headline": ms2 / ms_step}
+        compare.append({"precision": prec, "ms_per_step": ms2, "value": svs / (ms2 / 1e3),
+                        "main_kernel_ms": main2, "speedup_of_headline": ms2 / ms_step})
 
     # roofline of the dominant kernel: algorithmic bytes = packed .bed +
     # one f32 weight matrix + f64 scores and u32 counts

@@ -79,8 +79,12 @@ __device__ __forceinline__ uint32_t miss_i8(uint32_t sel) {
 
 }  // namespace
 
+#ifndef GNM_I8_WPE
+#define GNM_I8_WPE 1     // minimum waves per SIMD the register allocation must allow
+#endif
+
 template <int TMAX, int WAVES, int D, bool VEC, bool IDENT>
-__global__ __launch_bounds__(64 * WAVES) void k_score_i8(DevicePlan plan, const uint8_t *__restrict__ rows,
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(GNM_I8_WPE))) void k_score_i8(DevicePlan plan, const uint8_t *__restrict__ rows,
                                                          uint64_t stride, uint64_t bpv, uint64_t v0, uint64_t p0,
                                                          uint64_t p1, uint64_t split_base, uint64_t rps,
                                                          uint32_t splits, uint32_t person_tiles,
@@ -92,8 +96,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_score_i8(DevicePlan plan, const 
     constexpr int kTileWords = kCols * kColPitch / 4;
     constexpr int kNB = (TMAX + WAVES - 1) / WAVES;  // B chunks per thread per stage (some idle when TMAX < WAVES)
     static_assert(kFlushRows % (D * kI8Stage) == 0, "scale blocks must hold whole ring turns");
-    __shared__ uint32_t tile[2][kTileWords];
-    __shared__ u32x4_t btile[2][TMAX * 64];
+    // one LDS tile per stage of a batch: all D stages are stored, then one
+    // barrier, then all D are consumed (two barriers per batch of D stages)
+    __shared__ uint32_t tile[D][kTileWords];
+    __shared__ u32x4_t btile[D][TMAX * 64];
     __shared__ int stagebuf[WAVES][64 * 16];
 
     // XCD-aware placement, as in the split-BF16 kernel
@@ -337,17 +343,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_score_i8(DevicePlan plan, const 
         [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
             (issue(I, st0 + (i0 + I) * kI8Stage), ...);
         }(std::make_integer_sequence<int, D>{});
-        auto step = [&](auto dc) __attribute__((always_inline)) {
-            constexpr int d = decltype(dc)::value;
-            if (i0 + d >= nst) return;
-            const int buf = (int)((i0 + d) & 1);
-            store(buf, d);
-            lds_barrier_i8();
-            compute(buf);
-        };
         [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
-            (step(std::integral_constant<int, I>{}), ...);
+            ((i0 + I < nst ? (store(I, I), 0) : 0), ...);
         }(std::make_integer_sequence<int, D>{});
+        lds_barrier_i8();
+        [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
+            ((i0 + I < nst ? (compute(I), 0) : 0), ...);
+        }(std::make_integer_sequence<int, D>{});
+        lds_barrier_i8();   // every wave is done with the tiles before the next batch stores
         const uint64_t st_next = st0 + (i0 + D) * kI8Stage;
         if ((st_next % kFlushRows) == 0 || i0 + D >= nst) {
             flush((st0 + i0 * kI8Stage) / kFlushRows, first, false);
@@ -388,7 +391,8 @@ hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t
             if (identity) GNM_LAUNCH_I8(TM, WV, DD, false, true); else GNM_LAUNCH_I8(TM, WV, DD, false, false); \
         }                                                                                                      \
     } while (0)
-        if (nt <= 4) GNM_LAUNCH_I8_V(4, GNM_I8_WAVES, GNM_I8_BATCH);
+        if (nt <= 3) GNM_LAUNCH_I8_V(3, GNM_I8_WAVES, GNM_I8_BATCH);
+        else if (nt <= 4) GNM_LAUNCH_I8_V(4, GNM_I8_WAVES, GNM_I8_BATCH);
         else GNM_LAUNCH_I8_V(kMfmaMaxTiles, GNM_I8_WAVES, GNM_I8_BATCH);
 #undef GNM_LAUNCH_I8_V
 #undef GNM_LAUNCH_I8
