@@ -1,0 +1,77 @@
+"""Autotuner for the launch shape of the split-int8 scoring kernel.
+
+The kernel has two shape parameters (gnm_set_launch_config):
+  * tiles_per_pass -- 16-column weight/count tiles one workgroup accumulates
+    (3, 4 or 8).  More tiles per workgroup decode each .bed stage once for
+    more columns but hold more accumulators (fewer waves per SIMD); fewer
+    tiles give more workgroups per CU and more decode work.  All tile groups
+    run in one launch either way.
+  * batch_stages -- 64-row stages whose loads are issued together (2 or 4).
+
+Which shape wins depends on the number of people, the number of tiles
+(scores x limbs + membership classes) and the row pattern, so the tuner
+measures: each candidate scores the same sample block of rows on the
+context itself (HIP-event time of the main kernel, best of `reps`), and the
+fastest is kept.  The context must be fresh (no rows scored yet); its
+accumulators are restored exactly afterwards (they hold only the baseline
+term, b - b = 0).  Choices are remembered per shape class in memory and,
+when a path is given, in a JSON file.
+"""
+from __future__ import annotations
+
+import json
+import os
+
+CANDIDATES = [(3, 4), (3, 2), (4, 4), (4, 2), (8, 4), (8, 2)]
+
+_memory: dict[str, tuple[int, int]] = {}
+
+
+def shape_key(num_people: int, num_scores: int, precision: str, num_rows: int) -> str:
+    """Shape class: people and rows rounded to a power of two, exact scores."""
+    def p2(x):
+        return 1 << max(0, int(x - 1).bit_length())
+    return f"N{p2(num_people)}-K{num_scores}-{precision}-R{p2(num_rows)}"
+
+
+def _load(path):
+    if path and os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def tune(ctx, d_rows_ptr: int, row_stride: int, num_rows: int, first_variant: int = 0, reps: int = 2,
+         cache_path: str | None = None, candidates=CANDIDATES, log=None) -> dict:
+    """Pick the fastest launch shape for `ctx` on rows [first_variant,
+    first_variant + num_rows) of a device-resident .bed and apply it."""
+    key = shape_key(ctx.N, ctx.K, ctx.precision, ctx.V)
+    cache = _load(cache_path)
+    if key in _memory or key in cache:
+        choice = tuple(_memory.get(key) or cache[key]["choice"])
+        ctx.set_launch_config(*choice)
+        return {"key": key, "choice": list(choice), "cached": True}
+    S0, C0, done = ctx.read_accumulators()
+    if done != 0:
+        raise ValueError("autotune needs a fresh context (no rows scored yet)")
+    timings = {}
+    for tp, bs in candidates:
+        ctx.set_launch_config(tp, bs)
+        best = None
+        for r in range(reps + 1):           # the first run warms caches
+            ctx.score_rows_device(d_rows_ptr, row_stride, first_variant, num_rows)
+            main_ms, _ = ctx.last_timing()
+            if r:
+                best = main_ms if best is None else min(best, main_ms)
+        timings[f"{tp}x{bs}"] = best
+        if log:
+            log(f"autotune {key}: tiles_per_pass={tp} batch_stages={bs}: {best:.3f} ms")
+    ctx.seed_accumulators(S0, C0, 0)        # exact: baseline - baseline = 0
+    choice = min(candidates, key=lambda c: timings[f"{c[0]}x{c[1]}"])
+    ctx.set_launch_config(*choice)
+    _memory[key] = choice
+    if cache_path:
+        cache[key] = {"choice": list(choice), "timings_ms": timings, "sample_rows": num_rows}
+        with open(cache_path, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
+    return {"key": key, "choice": list(choice), "timings_ms": timings, "sample_rows": num_rows, "cached": False}

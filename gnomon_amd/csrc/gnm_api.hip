@@ -119,6 +119,8 @@ struct gnm_ctx {
     hipEvent_t ev_main0 = nullptr, ev_main1 = nullptr, ev_all0 = nullptr, ev_all1 = nullptr;
     bool timing_valid = false;
     uint64_t completed = 0;
+    uint32_t tiles_per_pass = 0;   // int8 kernel launch shape (0 = default), gnm_set_launch_config
+    uint32_t batch_stages = 0;
 
     template <class T>
     gnm_status alloc(T **out, size_t count) {
@@ -286,9 +288,11 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
             if (co != 0.0) need_e = true;
             if (we + 0.5 * co != 0.0) need_d = true;
         }
-        uint8_t m = need_e ? (need_d ? 3 : 2) : 1;
+        // a variant without CSR entries contributes nothing: no program row
+        const bool empty = c->row_offsets[v] == c->row_offsets[v + 1];
+        uint8_t m = empty ? 0 : (need_e ? (need_d ? 3 : 2) : 1);
         var_rows[v] = m;
-        c->var_prog_off[v + 1] = c->var_prog_off[v] + (m == 3 ? 2 : 1);
+        c->var_prog_off[v + 1] = c->var_prog_off[v] + (m == 3 ? 2 : (m ? 1 : 0));
     }
     const uint64_t R = c->var_prog_off[V];
     c->R = R;
@@ -312,6 +316,7 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
     // fill padding rows with the last real variant (never read: masked by range)
     for (uint64_t r = R; r < Rpad; ++r) prog_var[r] = R ? prog_var[R - 1] : 0;
     for (uint64_t v = 0; v < V; ++v) {
+        if (!var_rows[v]) continue;
         const uint64_t r = c->var_prog_off[v];
         uint32_t m = 0;
         for (uint64_t e = c->row_offsets[v]; e < c->row_offsets[v + 1]; ++e) {
@@ -412,44 +417,47 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
                 }
             });
         }
-        // Value of fragment element (row r, column tile*16 + n): a limb digit of
-        // the effect-allele weight, or 1 on count rows of the class column.
-        auto frag_value = [&](uint64_t r, uint32_t tile, uint32_t n, uint64_t blk, int32_t *dig) -> int32_t {
-            if (r >= R) return 0;
-            if (tile < pl.wtiles) {
-                const uint32_t colw = tile * 16 + n;
-                if (colw >= pl.ncol) return 0;
-                const uint32_t limb = colw / (uint32_t)K;
-                const uint64_t k = colw % K;
-                const double w = wrow[r * K + k];
-                if (w == 0.0) return 0;
-                const double Qd = std::nearbyint(std::ldexp(w, -uexp[blk * K + k]));
-                limb_digits((int64_t)Qd, L, dig);
-                return dig[limb];
-            }
-            const uint32_t cl = (tile - pl.wtiles) * 16 + n;
-            if (cl < c->nclass && prog_count_row[r]) {
-                const uint64_t v = prog_var[r];
-                for (uint64_t e = c->row_offsets[v]; e < c->row_offsets[v + 1]; ++e)
-                    if (c->score_class[c->cols[e]] == cl) return 1;
-            }
-            return 0;
-        };
         if (int8) {
             // [ks][tile][lane][16 i8]; lane (n = lane%16, g = lane/16) byte b
-            // holds row 64ks + 16g + b of column tile*16 + n.
+            // holds row 64ks + 16g + b of column tile*16 + n.  One pass over
+            // (row, score): the digits of a weight are computed once and each
+            // limb placed in its column; count rows set their class columns.
             std::vector<uint8_t> frag8(nks * tiles * 64 * 16, 0);
+            const uint32_t cw = (c->nclass + 31) / 32;
+            std::vector<uint32_t> rowcls(pl.ctiles ? R * cw : 0, 0);
+            if (pl.ctiles)
+                for (uint64_t v = 0; v < V; ++v) {
+                    if (!var_rows[v]) continue;
+                    const uint64_t r = c->var_prog_off[v];   // count row of the variant
+                    for (uint64_t e = c->row_offsets[v]; e < c->row_offsets[v + 1]; ++e) {
+                        const uint32_t cl = c->score_class[c->cols[e]];
+                        rowcls[r * cw + cl / 32] |= 1u << (cl % 32);
+                    }
+                }
             parallel_for(nks, [&](uint64_t a, uint64_t b) {
                 std::vector<int32_t> dig(L);
                 for (uint64_t ks = a; ks < b; ++ks) {
                     const uint64_t blk = ks * kstep / kFlushRows;
-                    for (uint32_t tile = 0; tile < tiles; ++tile)
-                        for (int lane = 0; lane < 64; ++lane)
-                            for (int j = 0; j < 16; ++j) {
-                                const uint64_t r = ks * kstep + (lane >> 4) * 16 + j;
-                                const int32_t v = frag_value(r, tile, lane & 15, blk, dig.data());
-                                frag8[((ks * tiles + tile) * 64 + lane) * 16 + j] = (uint8_t)(int8_t)v;
-                            }
+                    for (uint32_t rr = 0; rr < kstep; ++rr) {
+                        const uint64_t r = ks * kstep + rr;
+                        if (r >= R) break;
+                        const uint32_t g = rr / 16, byte = rr % 16;
+                        auto put = [&](uint32_t col, int32_t v) {
+                            const uint32_t tile = col / 16, lane = (col % 16) + 16 * g;
+                            frag8[((ks * tiles + tile) * 64 + lane) * 16 + byte] = (uint8_t)(int8_t)v;
+                        };
+                        for (uint64_t k = 0; k < K; ++k) {
+                            const double w = wrow[r * K + k];
+                            if (w == 0.0) continue;
+                            const double Qd = std::nearbyint(std::ldexp(w, -uexp[blk * K + k]));
+                            limb_digits((int64_t)Qd, L, dig.data());
+                            for (uint32_t l = 0; l < L; ++l) put(l * (uint32_t)K + (uint32_t)k, dig[l]);
+                        }
+                        if (pl.ctiles && prog_count_row[r])
+                            for (uint32_t wd = 0; wd < cw; ++wd)
+                                for (uint32_t bits = rowcls[r * cw + wd]; bits; bits &= bits - 1)
+                                    put(pl.wtiles * 16 + wd * 32 + (uint32_t)__builtin_ctz(bits), 1);
+                    }
                 }
             });
             uint8_t *d_frag8;
@@ -589,6 +597,14 @@ gnm_status score_device_rows(gnm_ctx *c, const uint8_t *d_rows, uint64_t stride,
     }
     const uint64_t p0 = c->var_prog_off[v0], p1 = c->var_prog_off[v0 + n];
     const uint32_t S = c->splits;
+    if (p1 == p0) {   // no variant of the block is in any score: nothing to add
+        GNM_HIP(hipEventRecord(c->ev_main0, c->stream));
+        GNM_HIP(hipEventRecord(c->ev_main1, c->stream));
+        GNM_HIP(hipEventRecord(c->ev_all1, c->stream));
+        c->timing_valid = true;
+        c->completed = v0 + n;
+        return GNM_OK;
+    }
     const bool identity = (p1 - p0) == n;  // no mixed-flip variant in this block
     GNM_HIP(hipEventRecord(c->ev_main0, c->stream));
     if (c->precision == GNM_PRECISION_FP32) {
@@ -599,7 +615,8 @@ gnm_status score_device_rows(gnm_ctx *c, const uint8_t *d_rows, uint64_t stride,
                                       c->plan.ctiles, identity, c->d_part, c->d_cpart, c->stream));
     } else if (c->plan.int8) {
         GNM_HIP(launch_score_i8(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, S, 0,
-                                c->plan.wtiles + c->plan.ctiles, identity, c->d_part, c->d_cpart, c->stream));
+                                c->plan.wtiles + c->plan.ctiles, identity, c->d_part, c->d_cpart, c->stream,
+                                c->tiles_per_pass, c->batch_stages));
     } else {
         GNM_HIP(launch_score_mfma(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, S, 0,
                                   c->plan.wtiles + c->plan.ctiles, identity, c->d_part, c->d_cpart, c->stream));
@@ -840,6 +857,17 @@ gnm_status gnm_unpack_device(const uint8_t *d_rows, uint64_t bytes_per_variant, 
     } catch (...) {
         return fail(GNM_ERR_INTERNAL, "exception in gnm_unpack_device");
     }
+}
+
+gnm_status gnm_set_launch_config(gnm_ctx *ctx, uint32_t tiles_per_pass, uint32_t batch_stages) {
+    if (!ctx) return fail(GNM_ERR_INVALID_ARG, "ctx is NULL");
+    if (tiles_per_pass != 0 && tiles_per_pass != 3 && tiles_per_pass != 4 && tiles_per_pass != 8)
+        return fail(GNM_ERR_INVALID_ARG, "tiles_per_pass must be 0, 3, 4 or 8");
+    if (batch_stages != 0 && batch_stages != 2 && batch_stages != 4)
+        return fail(GNM_ERR_INVALID_ARG, "batch_stages must be 0, 2 or 4");
+    ctx->tiles_per_pass = tiles_per_pass;
+    ctx->batch_stages = batch_stages;
+    return GNM_OK;
 }
 
 gnm_status gnm_synth_rows_device(uint64_t seed, uint32_t p_lo24, uint32_t p_hi24, uint32_t miss16,

@@ -88,8 +88,9 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(GNM_
                                                          uint64_t stride, uint64_t bpv, uint64_t v0, uint64_t p0,
                                                          uint64_t p1, uint64_t split_base, uint64_t rps,
                                                          uint32_t splits, uint32_t person_tiles,
-                                                         uint64_t num_people, uint32_t tile_begin,
-                                                         uint32_t ntile, double *__restrict__ part,
+                                                         uint64_t num_people, uint32_t tile_begin0,
+                                                         uint32_t tile_end, uint32_t ngroups,
+                                                         double *__restrict__ part,
                                                          uint32_t *__restrict__ cpart) {
     constexpr int kBytes = 16 * WAVES;               // packed bytes per row per workgroup
     constexpr int kCols = kBytes + 1;                // + the selector column
@@ -102,12 +103,19 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(GNM_
     __shared__ u32x4_t btile[D][TMAX * 64];
     __shared__ int stagebuf[WAVES][64 * 16];
 
-    // XCD-aware placement, as in the split-BF16 kernel
+    // Workgroups are dispatched round-robin over the 8 XCDs (L mod 8).  One
+    // launch covers every tile group: the groups of one (split, person tile)
+    // are consecutive slots of the same XCD, so they read the same .bed rows
+    // through that XCD's L2 at about the same time, and every workgroup of a
+    // split stays on one XCD (its B fragments are shared through that L2).
     const uint32_t L = blockIdx.x;
     const uint32_t xcd = L & 7u, slot = L >> 3;
-    const uint32_t s = xcd + 8u * (slot / person_tiles);
-    const uint32_t ptile = slot % person_tiles;
+    const uint32_t group = slot % ngroups, rest = slot / ngroups;
+    const uint32_t s = xcd + 8u * (rest / person_tiles);
+    const uint32_t ptile = rest % person_tiles;
     if (s >= splits) return;
+    const uint32_t tile_begin = tile_begin0 + group * TMAX;
+    const uint32_t ntile = tile_end - tile_begin < (uint32_t)TMAX ? tile_end - tile_begin : (uint32_t)TMAX;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -359,47 +367,49 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(GNM_
     }
 }
 
-#ifndef GNM_I8_BATCH
-#define GNM_I8_BATCH 4   // stages whose loads are issued together
-#endif
-#ifndef GNM_I8_WAVES
-#define GNM_I8_WAVES 4   // waves per workgroup (64 people each)
-#endif
-
 hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t stride, uint64_t bpv, uint64_t v0,
                            uint64_t p0, uint64_t p1, uint64_t num_people, uint32_t splits, uint32_t tile_begin,
-                           uint32_t tile_end, bool identity, double *part, uint32_t *cpart, hipStream_t stream) {
+                           uint32_t tile_end, bool identity, double *part, uint32_t *cpart, hipStream_t stream,
+                           uint32_t tiles_per_pass, uint32_t batch_stages) {
+    if (tile_end <= tile_begin) return hipSuccess;
     const bool vec = ((reinterpret_cast<uintptr_t>(rows) | stride) & 3u) == 0 && bpv >= 4;
     const uint64_t split_base = (p0 / kI8Stage) * kI8Stage;
     uint64_t rows_per_split = (p1 - split_base + splits - 1) / splits;
     rows_per_split = (rows_per_split + kI8Stage - 1) / kI8Stage * kI8Stage;
-    for (uint32_t tb = tile_begin; tb < tile_end; tb += kMfmaMaxTiles) {
-        const uint32_t nt = (tile_end - tb) < (uint32_t)kMfmaMaxTiles ? (tile_end - tb) : (uint32_t)kMfmaMaxTiles;
-#define GNM_LAUNCH_I8(TM, WV, DD, VC, ID)                                                                      \
-    do {                                                                                                       \
-        const uint32_t pt = (uint32_t)((num_people + 64 * WV - 1) / (64 * WV));                                \
-        const uint64_t wgs = (uint64_t)pt * splits;                                                            \
-        hipLaunchKernelGGL((k_score_i8<TM, WV, DD, VC, ID>), dim3((unsigned)wgs), dim3(64 * WV), 0, stream,      \
-                           plan, rows, stride, bpv, v0, p0, p1, split_base, rows_per_split, splits, pt,        \
-                           num_people, tb, nt, part, cpart);                                                   \
+    const uint32_t ntiles = tile_end - tile_begin;
+    // tiles per workgroup: 3 keeps three workgroups per CU (the 3-tile case
+    // of 8 scores with four limbs plus a count tile); larger counts use 4 or 8
+    uint32_t tmax = tiles_per_pass ? tiles_per_pass : (ntiles <= 3 ? 3u : (ntiles <= 4 ? 4u : 8u));
+    if (tmax != 3 && tmax != 4 && tmax != 8) return hipErrorInvalidValue;
+    const uint32_t depth = batch_stages ? batch_stages : 4u;
+    if (depth != 2 && depth != 4) return hipErrorInvalidValue;
+    const uint32_t ngroups = (ntiles + tmax - 1) / tmax;
+    constexpr int WV = 4;
+    const uint32_t pt = (uint32_t)((num_people + 64 * WV - 1) / (64 * WV));
+    const uint64_t wgs = (uint64_t)pt * splits * ngroups;   /* splits is a multiple of 8 */
+#define GNM_LAUNCH_I8(TM, DD, VC, ID)                                                                        \
+    hipLaunchKernelGGL((k_score_i8<TM, WV, DD, VC, ID>), dim3((unsigned)wgs), dim3(64 * WV), 0, stream, plan,   \
+                       rows, stride, bpv, v0, p0, p1, split_base, rows_per_split, splits, pt, num_people,       \
+                       tile_begin, tile_end, ngroups, part, cpart)
+#define GNM_LAUNCH_I8_VI(TM, DD)                                                                             \
+    do {                                                                                                     \
+        if (vec) {                                                                                           \
+            if (identity) GNM_LAUNCH_I8(TM, DD, true, true); else GNM_LAUNCH_I8(TM, DD, true, false);         \
+        } else {                                                                                             \
+            if (identity) GNM_LAUNCH_I8(TM, DD, false, true); else GNM_LAUNCH_I8(TM, DD, false, false);       \
+        }                                                                                                    \
     } while (0)
-#define GNM_LAUNCH_I8_V(TM, WV, DD)                                                                            \
-    do {                                                                                                       \
-        if (vec) {                                                                                             \
-            if (identity) GNM_LAUNCH_I8(TM, WV, DD, true, true); else GNM_LAUNCH_I8(TM, WV, DD, true, false);   \
-        } else {                                                                                               \
-            if (identity) GNM_LAUNCH_I8(TM, WV, DD, false, true); else GNM_LAUNCH_I8(TM, WV, DD, false, false); \
-        }                                                                                                      \
+#define GNM_LAUNCH_I8_D(TM)                                                                                  \
+    do {                                                                                                     \
+        if (depth == 2) GNM_LAUNCH_I8_VI(TM, 2); else GNM_LAUNCH_I8_VI(TM, 4);                               \
     } while (0)
-        if (nt <= 3) GNM_LAUNCH_I8_V(3, GNM_I8_WAVES, GNM_I8_BATCH);
-        else if (nt <= 4) GNM_LAUNCH_I8_V(4, GNM_I8_WAVES, GNM_I8_BATCH);
-        else GNM_LAUNCH_I8_V(kMfmaMaxTiles, GNM_I8_WAVES, GNM_I8_BATCH);
-#undef GNM_LAUNCH_I8_V
+    if (tmax == 3) GNM_LAUNCH_I8_D(3);
+    else if (tmax == 4) GNM_LAUNCH_I8_D(4);
+    else GNM_LAUNCH_I8_D(8);
+#undef GNM_LAUNCH_I8_D
+#undef GNM_LAUNCH_I8_VI
 #undef GNM_LAUNCH_I8
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

@@ -68,9 +68,13 @@ hipError_t launch_score_mfma(const DevicePlan &plan, const uint8_t *rows, uint64
                              uint32_t *cpart, hipStream_t stream);
 // Split-int8 scoring (gnm_i8.hip); rows must be 4-byte aligned with a
 // 4-byte multiple stride for the dword staging loads (else byte loads).
+// One launch covers every tile group of [tile_begin, tile_end) (groups of
+// tiles_per_pass tiles: 0 = default, else 4 or 8; batch_stages: 0 = default,
+// else 2 or 4 stages whose loads are issued together).
 hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t stride, uint64_t bpv, uint64_t v0,
                            uint64_t p0, uint64_t p1, uint64_t num_people, uint32_t splits, uint32_t tile_begin,
-                           uint32_t tile_end, bool identity, double *part, uint32_t *cpart, hipStream_t stream);
+                           uint32_t tile_end, bool identity, double *part, uint32_t *cpart, hipStream_t stream,
+                           uint32_t tiles_per_pass = 0, uint32_t batch_stages = 0);
 hipError_t launch_forward_frag_i8(const float *W, uint64_t V, uint64_t K, uint32_t L, const int *exps,
                                   uint32_t wtiles, uint32_t tiles, uint64_t nks, uint32_t *frag, double *scales,
                                   hipStream_t stream);

@@ -35,6 +35,7 @@ class ScoringContext:
                  out_to_fam=None, precision: str = "bf16x3", device: int = 0, force: bool = False):
         L = lib()
         self.csr = csr
+        self.precision = precision
         self.N = int(num_people)
         self.Nfam = int(total_people_in_fam if total_people_in_fam is not None else num_people)
         self.K = csr.num_scores
@@ -76,6 +77,10 @@ class ScoringContext:
     def score_rows_device(self, d_rows_ptr: int, row_stride: int, first_variant: int, num_rows: int) -> None:
         check(lib().gnm_score_rows_device(self._ctx, ctypes.c_void_p(d_rows_ptr), row_stride,
                                           first_variant, num_rows))
+
+    def set_launch_config(self, tiles_per_pass: int = 0, batch_stages: int = 0) -> None:
+        """Launch shape of the split-int8 kernel (results do not depend on it)."""
+        check(lib().gnm_set_launch_config(self._ctx, tiles_per_pass, batch_stages))
 
     def last_timing(self):
         a, b = ctypes.c_float(), ctypes.c_float()

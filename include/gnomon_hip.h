@@ -190,6 +190,17 @@ gnm_status gnm_synth_rows_device(uint64_t seed, uint32_t p_lo24, uint32_t p_hi24
                                  uint64_t num_rows, uint8_t *d_out, uint64_t row_stride,
                                  void *stream);
 
+/* Launch shape of the split-int8 scoring kernel on this context (tuning; no
+ * reference counterpart, used by the project's autotuner):
+ *   tiles_per_pass: 16-column weight/count tiles each workgroup accumulates
+ *                   (0 = default, 3, 4 or 8; all tile groups of a block still
+ *                   run in one launch);
+ *   batch_stages:   64-row stages whose loads are issued together (0 =
+ *                   default, 2 or 4).
+ * Results are identical for every shape (exact integer block sums, the same
+ * FP64 order). */
+gnm_status gnm_set_launch_config(gnm_ctx *ctx, uint32_t tiles_per_pass, uint32_t batch_stages);
+
 /* Kernel timing of the last gnm_score_rows[_device] call on this context,
  * measured with HIP events on the library's compute stream: milliseconds of
  * the main contraction kernel, and of the whole call (all kernels). */
