@@ -392,7 +392,8 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
     if (tiles) {
         const uint64_t kstep = int8 ? (uint64_t)kI8KStep : (uint64_t)kKStep;
         const uint64_t nks = Rpad / kstep;
-        const uint64_t nblk = (Rpad + kFlushRows - 1) / kFlushRows;
+        const uint64_t fr = (uint64_t)flush_rows(int8);   // rows per scale block
+        const uint64_t nblk = (Rpad + fr - 1) / fr;
         std::vector<double> scales(mfma_weights ? nblk * pl.ncol : 1, 0.0);
         // per (scale block, score): u = 2^e with max|w|/u <= 2^(8L) - 2^(8(L-1))
         std::vector<int> uexp(mfma_weights ? nblk * K : 1, 0);
@@ -400,7 +401,7 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
             const double qmax = limb_qmax(L, int8);
             parallel_for(nblk, [&](uint64_t a, uint64_t b) {
                 for (uint64_t blk = a; blk < b; ++blk) {
-                    const uint64_t r0 = blk * kFlushRows, r1 = std::min<uint64_t>(R, r0 + kFlushRows);
+                    const uint64_t r0 = blk * fr, r1 = std::min<uint64_t>(R, r0 + fr);
                     for (uint64_t k = 0; k < K; ++k) {
                         double m = 0.0;
                         for (uint64_t r = r0; r < r1; ++r) m = std::max(m, std::fabs(wrow[r * K + k]));
@@ -437,7 +438,7 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
             parallel_for(nks, [&](uint64_t a, uint64_t b) {
                 std::vector<int32_t> dig(L);
                 for (uint64_t ks = a; ks < b; ++ks) {
-                    const uint64_t blk = ks * kstep / kFlushRows;
+                    const uint64_t blk = ks * kstep / fr;
                     for (uint32_t rr = 0; rr < kstep; ++rr) {
                         const uint64_t r = ks * kstep + rr;
                         if (r >= R) break;
@@ -861,8 +862,9 @@ gnm_status gnm_unpack_device(const uint8_t *d_rows, uint64_t bytes_per_variant, 
 
 gnm_status gnm_set_launch_config(gnm_ctx *ctx, uint32_t tiles_per_pass, uint32_t batch_stages) {
     if (!ctx) return fail(GNM_ERR_INVALID_ARG, "ctx is NULL");
-    if (tiles_per_pass != 0 && tiles_per_pass != 3 && tiles_per_pass != 4 && tiles_per_pass != 8)
-        return fail(GNM_ERR_INVALID_ARG, "tiles_per_pass must be 0, 3, 4 or 8");
+    if (tiles_per_pass != 0 && tiles_per_pass != 3 && tiles_per_pass != 4 && tiles_per_pass != 6 &&
+        tiles_per_pass != 8)
+        return fail(GNM_ERR_INVALID_ARG, "tiles_per_pass must be 0, 3, 4, 6 or 8");
     if (batch_stages != 0 && batch_stages != 2 && batch_stages != 4)
         return fail(GNM_ERR_INVALID_ARG, "batch_stages must be 0, 2 or 4");
     ctx->tiles_per_pass = tiles_per_pass;
@@ -1003,14 +1005,15 @@ gnm_status gnm_dense_forward_device(const uint8_t *d_rows, uint64_t bytes_per_va
             pl.ctiles = 1;
             const uint32_t tiles = pl.wtiles + pl.ctiles;
             const uint64_t nks = Rpad / (int8 ? kI8KStep : kKStep);
-            const uint64_t nblk = (Rpad + kFlushRows - 1) / kFlushRows;
+            const uint64_t fr = (uint64_t)flush_rows(int8);
+            const uint64_t nblk = (Rpad + fr - 1) / fr;
             int *exps;
             uint16_t *frag;   // 16 bytes per (k-step, tile, lane) in both layouts
             double *scales;
             if ((s = ws_typed(ws, 8, nblk * K, st, &exps)) != GNM_OK) return s;
             if ((s = ws_typed(ws, 9, nks * tiles * 64 * 8, st, &frag)) != GNM_OK) return s;
             if ((s = ws_typed(ws, 10, nblk * pl.ncol, st, &scales)) != GNM_OK) return s;
-            GNM_HIP(launch_block_exponent(d_weights, V, K, K, kFlushRows, L, int8, exps, st));
+            GNM_HIP(launch_block_exponent(d_weights, V, K, K, fr, L, int8, exps, st));
             pl.scales = scales;
             pl.bfrag = (const uint32_t *)frag;
             if (int8) {

@@ -2,13 +2,13 @@
 // gnm_kernels.hip with weights written as signed 8-bit integer limbs and
 // v_mfma_i32_16x16x64_i8 (64 variants per MFMA, exact int32 accumulation).
 //
-// Per (scale block of 8192 program rows, weight column) the limb unit is a
-// power of two 2^e with max|w| <= 127 * 256^(L-1) * 2^e; w / 2^e is rounded
-// to an integer Q and written in L signed base-256 digits, every digit in
-// [-128, 127].  Dosages are 0/1/2, so each product is an exact small integer
-// and an int32 block sum is exact (8192 rows * 2 * 128 < 2^31); blocks are
-// flushed to FP64 with their scale.  L = 3 represents 23 bits, L = 4 31 bits
-// of the block maximum.
+// Per (scale block of kI8FlushRows = 65536 program rows, weight column) the
+// limb unit is a power of two 2^e with max|w| <= 127 * 256^(L-1) * 2^e;
+// w / 2^e is rounded to an integer Q and written in L signed base-256
+// digits, every digit in [-128, 127].  Dosages are 0/1/2, so each product is
+// an exact small integer and an int32 block sum is exact (65536 rows * 2 *
+// 128 = 2^24 < 2^31); blocks are flushed to FP64 with their scale.  L = 3
+// represents 23 bits, L = 4 31 bits of the block maximum.
 //
 // Data path, per workgroup of WAVES waves (64 people per wave, 16*WAVES
 // packed bytes of every row) and per stage of 64 program rows (one k-step):
@@ -28,9 +28,13 @@
 //        code 00 01 10 11  ->  dose 0 0 1 2 (A2 row) or 2 0 1 0 (A1 row),
 //                               missing 0 1 0 0;
 //   * M-tile t of the wave holds people 64w + 4i + t; B fragments (16 i8 per
-//     lane, the MFMA B operand) are staged through LDS per stage.
-// The register ring of D stages of unconditional loads and the LDS-only
-// stage barrier follow the split-BF16 kernel.
+//     lane, the MFMA B operand) go from memory to LDS by buffer LDS DMA,
+//     without passing through registers.
+// Loads of a batch of D stages are issued together (buffer loads: one
+// uniform descriptor per stage and a 32-bit lane offset, so no per-lane
+// 64-bit addresses stay live), stored, then consumed behind LDS-only
+// barriers.  The scale-block flush sits after the batch loop, so the
+// accumulators carried by that loop stay in accumulation registers.
 #include "gnm_internal.hpp"
 
 #include <utility>
@@ -44,6 +48,7 @@ namespace {
 
 constexpr int kI8Stage = 64;      // rows per stage = one k-step of v_mfma_i32_16x16x64_i8
 constexpr int kColPitch = 80;     // LDS bytes per tile column (64 rows + 16 pad)
+constexpr int kI8RsrcWord3 = 0x00020000;   // raw buffer descriptor (gfx9): 32-bit data format
 
 // LDS dword of (tile column col, row quad m): the four 16-row blocks of a
 // column are permuted by an XOR that depends on col / 16, so the transposed
@@ -53,8 +58,20 @@ __device__ __forceinline__ int tile_dword(int col, int m) {
     return col * (kColPitch / 4) + (m ^ (4 * ((col >> 4) & 3)));
 }
 
+// LDS written and read back by the same wave: LDS instructions of one wave
+// complete in order, so waiting for the writes is enough (no s_barrier).
+__device__ __forceinline__ void lds_wave_fence() {
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+}
+
 __device__ __forceinline__ void lds_barrier_i8() {
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Barrier after LDS DMA (global_load_lds): the DMA completes on the vector
+// memory counter, so every wave waits for its own transfers first.
+__device__ __forceinline__ void dma_lds_barrier() {
+    asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
 // 4x4 byte transpose: r[k] byte b -> c[b] byte k.
@@ -77,14 +94,16 @@ __device__ __forceinline__ uint32_t miss_i8(uint32_t sel) {
     return __builtin_amdgcn_perm(0x00000100u, 0x00000100u, sel);
 }
 
+// 16 bytes per lane from a buffer (lane offset voff) into LDS at
+// dst + 16 * lane (buffer_load_dwordx4 ... lds); dst must be wave-uniform.
+__device__ __forceinline__ void lds_dma_b128(__amdgpu_buffer_rsrc_t rsrc, void *dst, uint32_t voff) {
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (__attribute__((address_space(3))) void *)dst, 16, voff, 0, 0, 0);
+}
+
 }  // namespace
 
-#ifndef GNM_I8_WPE
-#define GNM_I8_WPE 1     // minimum waves per SIMD the register allocation must allow
-#endif
-
 template <int TMAX, int WAVES, int D, bool VEC, bool IDENT>
-__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(GNM_I8_WPE))) void k_score_i8(DevicePlan plan, const uint8_t *__restrict__ rows,
+__global__ __launch_bounds__(64 * WAVES) void k_score_i8(DevicePlan plan, const uint8_t *__restrict__ rows,
                                                          uint64_t stride, uint64_t bpv, uint64_t v0, uint64_t p0,
                                                          uint64_t p1, uint64_t split_base, uint64_t rps,
                                                          uint32_t splits, uint32_t person_tiles,
@@ -95,13 +114,12 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(GNM_
     constexpr int kBytes = 16 * WAVES;               // packed bytes per row per workgroup
     constexpr int kCols = kBytes + 1;                // + the selector column
     constexpr int kTileWords = kCols * kColPitch / 4;
-    constexpr int kNB = (TMAX + WAVES - 1) / WAVES;  // B chunks per thread per stage (some idle when TMAX < WAVES)
-    static_assert(kFlushRows % (D * kI8Stage) == 0, "scale blocks must hold whole ring turns");
+    constexpr int kBW = (TMAX + WAVES - 1) / WAVES;  // B tiles each wave fetches per stage
+    static_assert(kI8FlushRows % (D * kI8Stage) == 0, "scale blocks must hold whole batches");
     // one LDS tile per stage of a batch: all D stages are stored, then one
     // barrier, then all D are consumed (two barriers per batch of D stages)
     __shared__ uint32_t tile[D][kTileWords];
     __shared__ u32x4_t btile[D][TMAX * 64];
-    __shared__ int stagebuf[WAVES][64 * 16];
 
     // Workgroups are dispatched round-robin over the 8 XCDs (L mod 8).  One
     // launch covers every tile group: the groups of one (split, person tile)
@@ -118,7 +136,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(GNM_
     const uint32_t ntile = tile_end - tile_begin < (uint32_t)TMAX ? tile_end - tile_begin : (uint32_t)TMAX;
 
     const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int li = lane & 15, lg = lane >> 4;
     const uint64_t col0 = (uint64_t)ptile * kBytes;
     const uint64_t pw0 = (uint64_t)ptile * (64 * WAVES) + (uint64_t)wave * 64;
@@ -128,8 +147,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(GNM_
     const uint64_t r_end = p1 < se ? p1 : se;
     const uint32_t wt = plan.wtiles;
     const uint32_t tiles_total = plan.wtiles + plan.ctiles;
+    // weight tiles of this group (the first nw tiles); the others count missing calls
     const uint32_t nw = tile_begin < wt ? ((wt - tile_begin) < ntile ? (wt - tile_begin) : ntile) : 0;
-    const bool need_miss = nw < ntile;
     // load role: rows 4rg..4rg+3, packed bytes 4c4..4c4+3 of the workgroup's span
     // (consecutive threads take consecutive dwords of a row: each load
     // instruction reads 64 contiguous bytes of each of its rows)
@@ -144,10 +163,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(GNM_
     const uint32_t col_mask = ld_col + 4 <= bpv ? 0xFFFFFFFFu
                               : (ld_any ? (1u << (8u * (uint32_t)(bpv - ld_col))) - 1u : 0u);
     const u32x4_t *__restrict__ bfrag = (const u32x4_t *)plan.bfrag;
-    const uint32_t *__restrict__ fbw = (const uint32_t *)plan.flipbyte;
 
     uint32_t ra[D][4];      // row dwords of each stage in flight
-    u32x4_t rB[D][kNB];     // B chunks
     uint32_t rs[D];         // selector bytes of rows 4rg..4rg+3 (threads with c4 == 0)
     uint32_t rk[D];         // keep mask bits 0..3 for rows 4rg+k inside the split
     bool rfull[D];          // whole stage inside the split (uniform): no masking
@@ -155,25 +172,39 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(GNM_
     // identity block and the workgroup's byte span lies inside the row, so
     // the four row loads are one per-thread pointer plus uniform offsets.
     const bool span_full = VEC && col0 + kBytes <= bpv;
-    const uint8_t *bed_row0 = rows + (uint64_t)(4 * rg) * stride + (span_full ? ld_col : 0);
+    // per-lane byte offset of the fast path's first row dword from the
+    // stage's first row (the launcher keeps 64 rows x stride below 2^31)
+    const uint32_t row_voff = (uint32_t)(4 * (uint64_t)rg * stride + ld_col);
 
     const uint64_t last_st = ((r_end - 1) / kI8Stage) * kI8Stage;
-    uint32_t boff[kNB];
+    // B fragments: wave w fetches tiles w, w + WAVES, ... of the group by
+    // LDS DMA (lane l's 16 bytes land at btile[d][64 j + l]); tiles past
+    // ntile are fetched clamped to the last tile and their products are
+    // never flushed
+    uint32_t boff[kBW];     // byte offsets within a stage's fragment block
 #pragma unroll
-    for (int m = 0; m < kNB; ++m) {
-        const int q = tid + m * 64 * WAVES;
-        const int j = q / 64, l = q % 64;
-        // tiles >= ntile (and chunks past TMAX tiles) are loaded clamped and never read
-        const uint32_t jc = (uint32_t)j < ntile ? (uint32_t)j : ntile - 1;
-        boff[m] = (uint32_t)((tile_begin + jc) * 64 + l);
+    for (int m = 0; m < kBW; ++m) {
+        const uint32_t j = (uint32_t)(wave + m * WAVES);
+        const uint32_t jc = j < ntile ? j : ntile - 1;
+        boff[m] = ((tile_begin + jc) * 64 + lane) * 16;
     }
-    auto load_row = [&](uint64_t var) __attribute__((always_inline)) -> uint32_t {
+    auto fetch_b = [&](int d, uint64_t st) __attribute__((always_inline)) {
+        const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(bfrag + (st / kI8Stage) * tiles_total * 64), 0, 0x7FFFFFFF, kI8RsrcWord3);
+#pragma unroll
+        for (int m = 0; m < kBW; ++m) {
+            const int j = wave + m * WAVES;
+            if (kBW * WAVES == TMAX || j < TMAX)
+                lds_dma_b128(rb, &btile[d][j * 64], boff[m]);
+        }
+    };
+    auto load_row = [&](uint64_t var, uint64_t lcol) __attribute__((always_inline)) -> uint32_t {
         const uint8_t *src = rows + var * stride;
-        if (VEC) return *(const uint32_t *)(src + ld_colc);
+        if (VEC) return *(const uint32_t *)(src + (lcol + 4 <= bpv ? lcol : bpv - 4));
         uint32_t x = 0;
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const uint64_t o = ld_col + b;
+            const uint64_t o = lcol + b;
             const uint32_t byte = src[o < bpv ? o : bpv - 1];
             x |= (o < bpv ? byte : 0u) << (8 * b);
         }
@@ -181,35 +212,43 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(GNM_
     };
     auto issue = [&](int d, uint64_t st_req) __attribute__((always_inline)) {
         if (IDENT && span_full && st_req >= r_begin && st_req + kI8Stage <= r_end) {
-            const uint8_t *p = bed_row0 + (st_req - p0) * stride;
+            // buffer loads: a uniform descriptor per stage, one 32-bit lane
+            // offset, and the row step as the scalar offset
+            const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(rows + (st_req - p0) * stride), 0, 0x7FFFFFFF, kI8RsrcWord3);
 #pragma unroll
-            for (int k = 0; k < 4; ++k) ra[d][k] = *(const uint32_t *)(p + (uint64_t)k * stride);
+            for (int k = 0; k < 4; ++k)
+                ra[d][k] = __builtin_amdgcn_raw_buffer_load_b32(rr, row_voff, (uint32_t)(k * stride), 0);
             rk[d] = 0xFu;
             rfull[d] = true;
-            rs[d] = fbw[(st_req >> 2) + rg];
-            const u32x4_t *bs = bfrag + (st_req / kI8Stage) * tiles_total * 64;
-#pragma unroll
-            for (int m = 0; m < kNB; ++m) rB[d][m] = bs[boff[m]];
+            const __amdgpu_buffer_rsrc_t rf = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(plan.flipbyte + st_req), 0, 0x7FFFFFFF, kI8RsrcWord3);
+            rs[d] = __builtin_amdgcn_raw_buffer_load_b32(rf, 4u * (uint32_t)rg, 0, 0);
+            fetch_b(d, st_req);
             return;
         }
+        // Slow path (split edges, partial spans, non-identity rows): its
+        // per-lane inputs pass through an opaque copy, so what it derives
+        // from them is computed here and not kept live across the loop.
+        uint32_t rg4 = 4u * (uint32_t)rg;
+        uint64_t lcol = ld_col;
+        asm volatile("" : "+v"(rg4), "+v"(lcol));
         const uint64_t st = st_req < last_st ? st_req : last_st;
         const bool same = st == st_req;
         uint32_t keep = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const uint64_t r = st + 4 * rg + k;
-            const bool ok = same && r >= r_begin && r < r_end && ld_any;
+            const uint64_t r = st + rg4 + k;
+            const bool ok = same && r >= r_begin && r < r_end && lcol < bpv;
             keep |= ok ? (1u << k) : 0u;
             const uint64_t rc = r < r_begin ? r_begin : (r >= r_end ? r_end - 1 : r);
             const uint64_t var = IDENT ? (rc - p0) : (uint64_t)(plan.prog_var[rc] - (uint32_t)v0);
-            ra[d][k] = load_row(var);
+            ra[d][k] = load_row(var, lcol);
         }
         rk[d] = keep;
         rfull[d] = false;
-        rs[d] = fbw[(st >> 2) + rg];   // selector bytes of rows st+4rg..+3 (used by c4 == 0)
-        const u32x4_t *bs = bfrag + (st / kI8Stage) * tiles_total * 64;
-#pragma unroll
-        for (int m = 0; m < kNB; ++m) rB[d][m] = bs[boff[m]];
+        rs[d] = *(const uint32_t *)(plan.flipbyte + st + rg4);   // selector bytes of rows st+4rg..+3 (c4 == 0)
+        fetch_b(d, st);
     };
     auto store = [&](int buf, int d) __attribute__((always_inline)) {
         uint32_t r[4], c[4];
@@ -238,10 +277,6 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(GNM_
             }
             t[tile_dword(kBytes, rg)] = sel;
         }
-#pragma unroll
-        for (int m = 0; m < kNB; ++m)
-            if (kNB * 64 * WAVES == TMAX * 64 || tid + m * 64 * WAVES < TMAX * 64)
-                btile[buf][tid + m * 64 * WAVES] = rB[d][m];
     };
 
     i32x4_t acc[4][TMAX];
@@ -250,42 +285,62 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(GNM_
 #pragma unroll
         for (int j = 0; j < TMAX; ++j) acc[t][j] = i32x4_t{0, 0, 0, 0};
 
+    // Block flush: acc[t][j][q] is the int32 block sum of person
+    // pw0 + 16 lg + 4 q + t (MFMA output row 4 lg + q of M-tile t) and tile
+    // column li.  The wave stages one tile through its own LDS buffer so
+    // that each lane then owns one column for 16 people, and adds the values
+    // to the split partials in FP64 (weight tiles, times the block scale) or
+    // as counts.
+    __shared__ int stagebuf[WAVES][64 * 16];
     auto flush = [&](uint64_t blk, bool first, bool empty) __attribute__((always_inline)) {
+        // opaque to the optimiser, so the addresses below are formed here
+        // rather than hoisted out of the row loop into live registers
+        uint64_t prow = (uint64_t)s * num_people + pw0;
+        uint32_t ncol = plan.ncol, nclass = plan.nclass, tb = tile_begin, lane_c = (uint32_t)li;
+        asm volatile("" : "+v"(prow), "+v"(ncol), "+v"(nclass), "+v"(tb), "+v"(lane_c));
+        ncol = __builtin_amdgcn_readfirstlane(ncol);
+        nclass = __builtin_amdgcn_readfirstlane(nclass);
+        tb = __builtin_amdgcn_readfirstlane(tb);
+        int *sbuf = stagebuf[wave];
         auto flush_tile = [&](auto jc) __attribute__((always_inline)) {
             constexpr int j = decltype(jc)::value;
-            if ((uint32_t)j >= ntile) return;
-            lds_barrier_i8();
+            __builtin_amdgcn_sched_barrier(0);
+            // keeps this tile's sums in accumulation registers up to here,
+            // so only 16 of them are copied out at a time
+#pragma unroll
+            for (int t = 0; t < 4; ++t) asm volatile("" : "+a"(acc[t][j]));
+            lds_wave_fence();    // the previous tile's reads of sbuf are done
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) stagebuf[wave][(16 * lg + 4 * q + t) * 16 + li] = acc[t][j][q];
+                for (int q = 0; q < 4; ++q) sbuf[(16 * lg + 4 * q + t) * 16 + li] = acc[t][j][q];
                 acc[t][j] = i32x4_t{0, 0, 0, 0};
             }
-            lds_barrier_i8();
-            const uint32_t tl = tile_begin + j;
+            lds_wave_fence();
+            const uint32_t tl = tb + j;
             const bool wtile = tl < wt;
-            const uint32_t c = wtile ? tl * 16 + li : (tl - wt) * 16 + li;
-            const uint32_t ncols = wtile ? plan.ncol : plan.nclass;
-            if (c < ncols) {
-                const double sc = (wtile && !empty) ? plan.scales[blk * plan.ncol + c] : 0.0;
+            const uint32_t c = (wtile ? tl : tl - wt) * 16 + lane_c;
+            const uint32_t nc = wtile ? ncol : nclass;
+            if ((uint32_t)j < ntile && c < nc) {
+                const double sc = (wtile && !empty) ? plan.scales[blk * ncol + c] : 0.0;
 #pragma unroll 1
                 for (int i = 0; i < 16; ++i) {
                     const uint32_t pl = 4 * i + lg;  // person of the wave, 0..63
-                    const uint64_t p = pw0 + pl;
-                    if (p >= num_people) continue;
-                    const int v = stagebuf[wave][pl * 16 + li];
-                    const uint64_t row = (uint64_t)s * num_people + p;
+                    if (pw0 + pl >= num_people) continue;
+                    const int v = sbuf[pl * 16 + lane_c];
+                    const uint64_t row = prow + pl;
                     if (wtile) {
-                        double *dd = part + row * plan.ncol + c;
+                        double *dd = part + row * ncol + c;
                         const double x = (double)v * sc;
                         *dd = first ? x : *dd + x;
                     } else {
-                        uint32_t *dd = cpart + row * plan.nclass + c;
+                        uint32_t *dd = cpart + row * nclass + c;
                         const uint32_t x = (uint32_t)v;
                         *dd = first ? x : *dd + x;
                     }
                 }
             }
+            __builtin_amdgcn_sched_barrier(0);
         };
         [&]<int... J>(std::integer_sequence<int, J...>) __attribute__((always_inline)) {
             (flush_tile(std::integral_constant<int, J>{}), ...);
@@ -296,6 +351,9 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(GNM_
         const uint32_t *t = tile[buf];
         const u32x4_t X = *(const u32x4_t *)&t[tile_dword(wave * 16 + li, 4 * lg)];
         const u32x4_t F = *(const u32x4_t *)&t[tile_dword(kBytes, 4 * lg)];
+        u32x4_t bv[TMAX];
+#pragma unroll
+        for (int j = 0; j < TMAX; ++j) bv[j] = btile[buf][j * 64 + lane];
         uint32_t sel[4][4];   // [person t][dword q]
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -309,14 +367,13 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(GNM_
 #pragma unroll
         for (int j = 0; j < TMAX; ++j) {
             if ((uint32_t)j < nw) {
-                const u32x4_t bv = btile[buf][j * 64 + lane];
-                const i32x4_t B = i32x4_t{(int)bv[0], (int)bv[1], (int)bv[2], (int)bv[3]};
+                const i32x4_t B = i32x4_t{(int)bv[j][0], (int)bv[j][1], (int)bv[j][2], (int)bv[j][3]};
 #pragma unroll
                 for (int tt = 0; tt < 4; ++tt)
                     acc[tt][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[tt], B, acc[tt][j], 0, 0, 0);
             }
         }
-        if (need_miss) {
+        if (nw < ntile) {
             i32x4_t M[4];
 #pragma unroll
             for (int tt = 0; tt < 4; ++tt)
@@ -325,8 +382,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(GNM_
 #pragma unroll
             for (int j = 0; j < TMAX; ++j) {
                 if ((uint32_t)j >= nw && (uint32_t)j < ntile) {
-                    const u32x4_t bv = btile[buf][j * 64 + lane];
-                    const i32x4_t B = i32x4_t{(int)bv[0], (int)bv[1], (int)bv[2], (int)bv[3]};
+                    const i32x4_t B = i32x4_t{(int)bv[j][0], (int)bv[j][1], (int)bv[j][2], (int)bv[j][3]};
 #pragma unroll
                     for (int tt = 0; tt < 4; ++tt)
                         acc[tt][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(M[tt], B, acc[tt][j], 0, 0, 0);
@@ -346,25 +402,39 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(GNM_
     // allocator, and the copy waits for every load in flight).
     const uint64_t st0 = (r_begin / (D * kI8Stage)) * (D * kI8Stage);
     const uint64_t nst = (r_end - st0 + kI8Stage - 1) / kI8Stage;
+    // Outer loop over scale blocks, inner loop over batches of the block:
+    // the flush sits after the inner loop on every path, so the
+    // accumulators carried by the inner loop are only ever MFMA results and
+    // stay in accumulation registers.
     bool first = true;
-    for (uint64_t i0 = 0; i0 < nst; i0 += D) {
-        [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
-            (issue(I, st0 + (i0 + I) * kI8Stage), ...);
-        }(std::make_integer_sequence<int, D>{});
-        [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
-            ((i0 + I < nst ? (store(I, I), 0) : 0), ...);
-        }(std::make_integer_sequence<int, D>{});
-        lds_barrier_i8();
-        [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
-            ((i0 + I < nst ? (compute(I), 0) : 0), ...);
-        }(std::make_integer_sequence<int, D>{});
-        lds_barrier_i8();   // every wave is done with the tiles before the next batch stores
-        const uint64_t st_next = st0 + (i0 + D) * kI8Stage;
-        if ((st_next % kFlushRows) == 0 || i0 + D >= nst) {
-            flush((st0 + i0 * kI8Stage) / kFlushRows, first, false);
-            first = false;
-        }
+    for (uint64_t i0 = 0; i0 < nst;) {
+        const uint64_t blk = (st0 + i0 * kI8Stage) / kI8FlushRows;
+        const uint64_t blk_end = ((blk + 1) * kI8FlushRows - st0) / kI8Stage;   // a multiple of D
+        const uint64_t i_end = blk_end < nst ? blk_end : nst;
+        do {   // i0 < i_end on entry: the block holds stage i0
+            [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
+                (issue(I, st0 + (i0 + I) * kI8Stage), ...);
+            }(std::make_integer_sequence<int, D>{});
+            [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
+                ((i0 + I < nst ? (store(I, I), 0) : 0), ...);
+            }(std::make_integer_sequence<int, D>{});
+            dma_lds_barrier();   // B fragments landed and every tile stored
+            [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
+                ((i0 + I < nst ? (compute(I), 0) : 0), ...);
+            }(std::make_integer_sequence<int, D>{});
+            lds_barrier_i8();   // every wave is done with the tiles before the next batch stores
+            i0 += D;
+        } while (i0 < i_end);
+        flush(blk, first, false);
+        first = false;
     }
+}
+
+// Rough cost of one group pass per wave and stage, in cycles: decode VALU
+// (~60 instructions) plus 4 MFMA of 16 cycles per tile slot.
+static uint64_t i8_pass_cost(uint32_t ntiles, uint32_t tmax) {
+    const uint64_t groups = (ntiles + tmax - 1) / tmax;
+    return groups * (240u + 64u * tmax);
 }
 
 hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t stride, uint64_t bpv, uint64_t v0,
@@ -372,15 +442,22 @@ hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t
                            uint32_t tile_end, bool identity, double *part, uint32_t *cpart, hipStream_t stream,
                            uint32_t tiles_per_pass, uint32_t batch_stages) {
     if (tile_end <= tile_begin) return hipSuccess;
-    const bool vec = ((reinterpret_cast<uintptr_t>(rows) | stride) & 3u) == 0 && bpv >= 4;
+    // dword row loads need 4-byte alignment; the fast path's buffer offsets
+    // (up to 64 rows x stride) must stay below the descriptors' 2^31 - 1 range
+    const bool vec = ((reinterpret_cast<uintptr_t>(rows) | stride) & 3u) == 0 && bpv >= 4 && stride < (1ull << 25);
     const uint64_t split_base = (p0 / kI8Stage) * kI8Stage;
     uint64_t rows_per_split = (p1 - split_base + splits - 1) / splits;
     rows_per_split = (rows_per_split + kI8Stage - 1) / kI8Stage * kI8Stage;
     const uint32_t ntiles = tile_end - tile_begin;
-    // tiles per workgroup: 3 keeps three workgroups per CU (the 3-tile case
-    // of 8 scores with four limbs plus a count tile); larger counts use 4 or 8
-    uint32_t tmax = tiles_per_pass ? tiles_per_pass : (ntiles <= 3 ? 3u : (ntiles <= 4 ? 4u : 8u));
-    if (tmax != 3 && tmax != 4 && tmax != 8) return hipErrorInvalidValue;
+    // tiles per workgroup pass: every group computes tmax tile slots, so the
+    // default takes the cheapest of 3, 4, 6 and 8 by the cost model above
+    uint32_t tmax = tiles_per_pass;
+    if (!tmax) {
+        tmax = 3;
+        for (uint32_t c : {4u, 6u, 8u})
+            if (i8_pass_cost(ntiles, c) < i8_pass_cost(ntiles, tmax)) tmax = c;
+    }
+    if (tmax != 3 && tmax != 4 && tmax != 6 && tmax != 8) return hipErrorInvalidValue;
     const uint32_t depth = batch_stages ? batch_stages : 4u;
     if (depth != 2 && depth != 4) return hipErrorInvalidValue;
     const uint32_t ngroups = (ntiles + tmax - 1) / tmax;
@@ -405,6 +482,7 @@ hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t
     } while (0)
     if (tmax == 3) GNM_LAUNCH_I8_D(3);
     else if (tmax == 4) GNM_LAUNCH_I8_D(4);
+    else if (tmax == 6) GNM_LAUNCH_I8_D(6);
     else GNM_LAUNCH_I8_D(8);
 #undef GNM_LAUNCH_I8_D
 #undef GNM_LAUNCH_I8_VI
@@ -441,7 +519,7 @@ __global__ void k_forward_frag_i8(const float *__restrict__ W, uint64_t V, uint6
     const uint64_t ks = (id >> 6) / tiles;
     const uint32_t n = lane & 15, g = lane >> 4;
     const uint64_t ncol = (uint64_t)L * K;
-    const uint64_t blk = ks * kI8Stage / kFlushRows;
+    const uint64_t blk = ks * kI8Stage / kI8FlushRows;
     uint32_t out[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int b = 0; b < 16; ++b) {
@@ -464,7 +542,7 @@ __global__ void k_forward_frag_i8(const float *__restrict__ W, uint64_t V, uint6
     uint32_t *dst = frag + id * 4;
 #pragma unroll
     for (int q = 0; q < 4; ++q) dst[q] = out[q];
-    if (tile < wtiles && g == 0 && (ks * kI8Stage) % kFlushRows == 0) {
+    if (tile < wtiles && g == 0 && (ks * kI8Stage) % kI8FlushRows == 0) {
         const uint64_t c = (uint64_t)tile * 16 + n;
         if (c < ncol) {
             const uint32_t limb = (uint32_t)(c / K);

@@ -14,6 +14,12 @@ namespace gnm {
 constexpr int kKStep = 32;
 constexpr int kFlushRows = 8192;
 constexpr int kKStepsPerFlush = kFlushRows / kKStep;
+// The split-int8 path has its own, longer scale blocks: an int32 sum of
+// dosage (<= 2) x int8 limb (|.| <= 128) products stays exact up to 2^22
+// rows; 65536 rows (sums < 2^24) keep the FP64 read-modify-write flushes of
+// the partial sums rare.
+constexpr int kI8FlushRows = 65536;
+constexpr int flush_rows(bool int8) { return int8 ? kI8FlushRows : kFlushRows; }
 constexpr int kLimbBits = 8;
 constexpr int kFp32ScoreChunk = 8;     // scores per pass of the FP32 kernel
 constexpr int kFp32MaxClasses = 4;     // membership classes counted in-kernel by the FP32 kernel
@@ -30,7 +36,7 @@ struct DevicePlan {
     uint64_t k32;
     // split-BF16 path
     const uint32_t *bfrag;     // [R/32][tiles][64 lanes][4 dwords] B fragments (weights then counts)
-    const double *scales;      // [R/kFlushRows][ncol] power-of-two limb scales
+    const double *scales;      // [R/flush_rows(int8)][ncol] power-of-two limb scales
     uint32_t limbs;            // 2 or 3 (BF16 limbs), 3 or 4 (int8 limbs)
     uint32_t int8;             // 1: int8 limbs, fragments [R/64][tiles][64 lanes][16 i8]
     const uint8_t *flipbyte;   // [Rpad] int8 path: 4 for rows that read the A1 allele, else 0
@@ -67,10 +73,11 @@ hipError_t launch_score_mfma(const DevicePlan &plan, const uint8_t *rows, uint64
                              uint32_t tile_begin, uint32_t tile_end, bool identity, double *part,
                              uint32_t *cpart, hipStream_t stream);
 // Split-int8 scoring (gnm_i8.hip); rows must be 4-byte aligned with a
-// 4-byte multiple stride for the dword staging loads (else byte loads).
-// One launch covers every tile group of [tile_begin, tile_end) (groups of
-// tiles_per_pass tiles: 0 = default, else 4 or 8; batch_stages: 0 = default,
-// else 2 or 4 stages whose loads are issued together).
+// 4-byte multiple stride below 2^25 for the dword staging loads (else byte
+// loads).  One launch covers every tile group of [tile_begin, tile_end)
+// (groups of tiles_per_pass tiles: 0 = default, else 3, 4, 6 or 8;
+// batch_stages: 0 = default, else 2 or 4 stages whose loads are issued
+// together).
 hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t stride, uint64_t bpv, uint64_t v0,
                            uint64_t p0, uint64_t p1, uint64_t num_people, uint32_t splits, uint32_t tile_begin,
                            uint32_t tile_end, bool identity, double *part, uint32_t *cpart, hipStream_t stream,

@@ -24,6 +24,7 @@ from helpers import pack_bed
 pytestmark = pytest.mark.gpu
 
 BLOCK = 8192   # rows (forward: variants; adjoint: people) per scale block
+I8_BLOCK = 65536   # variants per scale block of the split-int8 forward
 LIMBS = {"bf16x3": 3, "bf16x2": 2, "i8x4": 4, "i8x3": 3}
 
 
@@ -53,15 +54,15 @@ def _effect_dosage(dos, flip):
     return g
 
 
-def _limb_bound(G, X, precision):
+def _limb_bound(G, X, precision, block=BLOCK):
     """max error of sum over rows r of G[r, :] * X[r, k] when X is limb-split
-    per BLOCK rows and column: sum_blk (sum_{r in blk} G[r, i]) * m_blk,k / qmax."""
+    per `block` rows and column: sum_blk (sum_{r in blk} G[r, i]) * m_blk,k / qmax."""
     d = _qmax(precision)
     R = X.shape[0]
     out = 0.0
-    for b0 in range(0, R, BLOCK):
-        m = np.abs(X[b0:b0 + BLOCK]).max(0)          # [K]
-        gsum = G[b0:b0 + BLOCK].sum(0)               # [I]
+    for b0 in range(0, R, block):
+        m = np.abs(X[b0:b0 + block]).max(0)          # [K]
+        gsum = G[b0:b0 + block].sum(0)               # [I]
         out = out + np.outer(gsum, m) / d
     return out
 
@@ -85,7 +86,8 @@ def test_dense_forward_matches_fp64(orc, precision):
         assert err.max() <= 1e-12 * np.abs(S64).max()
     else:
         G = _effect_dosage(dos, flip)                 # [V, N]
-        bound = _limb_bound(G, W.astype(np.float64), precision)
+        bound = _limb_bound(G, W.astype(np.float64), precision,
+                            I8_BLOCK if precision.startswith("i8") else BLOCK)
         assert np.all(err <= bound + 1e-12 * np.abs(S64).max()), (err - bound).max()
         if precision in ("bf16x3", "i8x4"):
             assert err.max() / np.abs(S64).max() <= 1e-6

@@ -290,10 +290,45 @@ def test_device_resident_rows_match_host_rows(orc):
     rows = orc.synth_rows(5, N, 0, V, 167772, 8388608, 1311)
     csr = csr_from_dense(rng.normal(0, 0.01, (V, K)).astype(np.float32), rng.random(V) < 0.5)
     S64, C64 = orc.score_fp64(rows, csr, N)
-    for precision in ("fp32", "bf16x3"):
+    for precision in ("fp32", "bf16x3", "i8x4"):
         with ScoringContext(csr, N, precision=precision, force=True) as ctx:
             ctx.score_rows_device(d.data_ptr(), stride, 0, V)
             S, C = ctx.finish()
             main_ms, total_ms = ctx.last_timing()
             assert 0 < main_ms <= total_ms
         _check_scores(S, C, S64, C64, precision)
+
+
+@pytest.mark.parametrize("precision", ["i8x4", "i8x3"])
+def test_int8_scale_blocks_and_launch_shapes(orc, precision):
+    """150,000 variants = two full int8 scale blocks of 65,536 rows and a
+    partial one; 700 people (ragged, one split per 2,344 rows, so splits start
+    and end inside blocks); 24 scores with per-cell presence (6 weight tiles +
+    a count tile: groups that mix weight and count tiles, count-only groups
+    and unused tile slots, depending on the shape).  Every launch shape gives
+    bit-identical results (exact integer block sums, the same FP64 order),
+    and they match the FP64 oracle."""
+    _flush(f"int8 blocks {precision}")
+    rng = np.random.default_rng(31)
+    N, V, K = 700, 150_000, 24
+    rows = orc.synth_rows(8, N, 3, V, int(0.02 * 2**24), int(0.5 * 2**24), 1311)
+    W = rng.normal(0, 0.01, (V, K)).astype(np.float32)
+    W[60_000:70_000] *= 1e-3            # one block with much smaller weights
+    present = rng.random((V, K)) < 0.9
+    present[:, :4] = True
+    csr = csr_from_dense(W, rng.random((V, K)) < 0.5, present)
+    S64, C64 = orc.score_fp64(rows, csr, N)
+    first = None
+    for tp, bs in [(0, 0), (3, 2), (3, 4), (4, 4), (6, 2), (6, 4), (8, 2), (8, 4)]:
+        _flush(f"  tiles_per_pass={tp} batch_stages={bs}")
+        with ScoringContext(csr, N, precision=precision, force=True) as ctx:
+            ctx.set_launch_config(tp, bs)
+            for a, b in [(0, 40_000), (40_000, 131_072), (131_072, V)]:
+                ctx.score_rows(rows[a:b], a)
+            S, C = ctx.finish()
+        if first is None:
+            first = (S, C)
+            _check_scores(S, C, S64, C64, precision)
+        else:
+            assert np.array_equal(C, first[1])
+            assert np.array_equal(S, first[0]), f"shape {tp}x{bs} differs"
