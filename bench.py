@@ -431,6 +431,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="check a slice against the FP64 oracle")
     ap.add_argument("--no-unpack", action="store_true", help="skip the standalone unpack-stage measurement")
+    ap.add_argument("--no-report", action="store_true", help="skip the per-score precision report (precision_64)")
+    ap.add_argument("--launch", default="", metavar="TP,BS",
+                    help="int8 launch shape (tiles per pass, batch stages) instead of the default")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise copy of MIB MiB (read + write) three times; used to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters")
@@ -477,6 +480,8 @@ def main():
     def run_path(precision):
         t1 = time.time()
         ctx = ScoringContext(csr, N, precision=precision, force=True)
+        if args.launch and precision.startswith("i8"):
+            ctx.set_launch_config(*(int(x) for x in args.launch.split(",")))
         log(f"[{precision}] plan built ({time.time() - t1:.1f}s)")
         for _ in range(args.warmup):
             ctx.score_rows_device(d_bed.data_ptr(), stride, 0, V)
@@ -535,7 +540,7 @@ def main():
                 "kernel": kname, "kernel_ms": round(main_ms, 4), "algorithmic_bytes": alg_bytes}
 
     precision_report = None
-    if w.name == "precision_64":
+    if w.name == "precision_64" and not args.no_report:
         precision_report = run_precision_report(w, csr, d_bed, stride, args)
     unpack = None if args.no_unpack or w.name != "plain_fp32_biobank" else measure_unpack(d_bed, N, V, bpv, stride, max(3, min(args.steps, 10)), w.name)
 

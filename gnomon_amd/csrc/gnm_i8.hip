@@ -71,6 +71,11 @@ __device__ __forceinline__ void lds_barrier_i8() {
 // Barrier after LDS DMA (global_load_lds): the DMA completes on the vector
 // memory counter, so every wave waits for its own transfers first.
 __device__ __forceinline__ void dma_lds_barrier() {
+    // the same wait as an instruction the compiler sees (gfx9 encoding:
+    // vmcnt 0, expcnt 7, lgkmcnt 0), so it knows the DMA has landed and
+    // adds no waits for it to later LDS reads; the asm repeats it so that
+    // no memory operation can slip between the wait and the barrier
+    __builtin_amdgcn_s_waitcnt(0x0070);
     asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
@@ -119,7 +124,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_score_i8(DevicePlan plan, const 
     // one LDS tile per stage of a batch: all D stages are stored, then one
     // barrier, then all D are consumed (two barriers per batch of D stages)
     __shared__ uint32_t tile[D][kTileWords];
-    __shared__ u32x4_t btile[D][TMAX * 64];
+    // B fragments are double-buffered: the next batch's DMA lands in one
+    // half while this batch's MFMAs read the other
+    __shared__ u32x4_t btile[2][D][TMAX * 64];
 
     // Workgroups are dispatched round-robin over the 8 XCDs (L mod 8).  One
     // launch covers every tile group: the groups of one (split, person tile)
@@ -188,14 +195,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_score_i8(DevicePlan plan, const 
         const uint32_t jc = j < ntile ? j : ntile - 1;
         boff[m] = ((tile_begin + jc) * 64 + lane) * 16;
     }
-    auto fetch_b = [&](int d, uint64_t st) __attribute__((always_inline)) {
+    auto fetch_b = [&](uint32_t par, int d, uint64_t st) __attribute__((always_inline)) {
         const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
             (void *)(bfrag + (st / kI8Stage) * tiles_total * 64), 0, 0x7FFFFFFF, kI8RsrcWord3);
 #pragma unroll
         for (int m = 0; m < kBW; ++m) {
             const int j = wave + m * WAVES;
             if (kBW * WAVES == TMAX || j < TMAX)
-                lds_dma_b128(rb, &btile[d][j * 64], boff[m]);
+                lds_dma_b128(rb, &btile[par][d][j * 64], boff[m]);
         }
     };
     auto load_row = [&](uint64_t var, uint64_t lcol) __attribute__((always_inline)) -> uint32_t {
@@ -210,7 +217,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_score_i8(DevicePlan plan, const 
         }
         return x;
     };
-    auto issue = [&](int d, uint64_t st_req) __attribute__((always_inline)) {
+    auto issue = [&](uint32_t par, int d, uint64_t st_req) __attribute__((always_inline)) {
         if (IDENT && span_full && st_req >= r_begin && st_req + kI8Stage <= r_end) {
             // buffer loads: a uniform descriptor per stage, one 32-bit lane
             // offset, and the row step as the scalar offset
@@ -224,7 +231,6 @@ __global__ __launch_bounds__(64 * WAVES) void k_score_i8(DevicePlan plan, const 
             const __amdgpu_buffer_rsrc_t rf = __builtin_amdgcn_make_buffer_rsrc(
                 (void *)(plan.flipbyte + st_req), 0, 0x7FFFFFFF, kI8RsrcWord3);
             rs[d] = __builtin_amdgcn_raw_buffer_load_b32(rf, 4u * (uint32_t)rg, 0, 0);
-            fetch_b(d, st_req);
             return;
         }
         // Slow path (split edges, partial spans, non-identity rows): its
@@ -248,7 +254,6 @@ __global__ __launch_bounds__(64 * WAVES) void k_score_i8(DevicePlan plan, const 
         rk[d] = keep;
         rfull[d] = false;
         rs[d] = *(const uint32_t *)(plan.flipbyte + st + rg4);   // selector bytes of rows st+4rg..+3 (c4 == 0)
-        fetch_b(d, st);
     };
     auto store = [&](int buf, int d) __attribute__((always_inline)) {
         uint32_t r[4], c[4];
@@ -279,11 +284,17 @@ __global__ __launch_bounds__(64 * WAVES) void k_score_i8(DevicePlan plan, const 
         }
     };
 
-    i32x4_t acc[4][TMAX];
+    typedef i32x4_t Acc[4][TMAX];   // int32 block sums [person tile t][tile j]
+    // zero from an opaque copy of 0: the initialisation stays in the branch
+    // that uses the set instead of being hoisted above the layout dispatch
+    auto zero_acc = [&](Acc &acc) __attribute__((always_inline)) {
+        int z = 0;
+        asm volatile("" : "+v"(z));
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int j = 0; j < TMAX; ++j) acc[t][j] = i32x4_t{0, 0, 0, 0};
+            for (int j = 0; j < TMAX; ++j) acc[t][j] = i32x4_t{z, z, z, z};
+    };
 
     // Block flush: acc[t][j][q] is the int32 block sum of person
     // pw0 + 16 lg + 4 q + t (MFMA output row 4 lg + q of M-tile t) and tile
@@ -291,8 +302,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_score_i8(DevicePlan plan, const 
     // that each lane then owns one column for 16 people, and adds the values
     // to the split partials in FP64 (weight tiles, times the block scale) or
     // as counts.
-    __shared__ int stagebuf[WAVES][64 * 16];
-    auto flush = [&](uint64_t blk, bool first, bool empty) __attribute__((always_inline)) {
+    // the staging buffer shares the .bed tiles' LDS when they are large
+    // enough (no tile is in use while a block is flushed)
+    constexpr bool kStageInTile = D * kTileWords >= WAVES * 64 * 16;
+    __shared__ int stage_own[kStageInTile ? 1 : WAVES * 64 * 16];
+    auto flush = [&](Acc &acc, uint64_t blk, bool first, bool empty) __attribute__((always_inline)) {
         // opaque to the optimiser, so the addresses below are formed here
         // rather than hoisted out of the row loop into live registers
         uint64_t prow = (uint64_t)s * num_people + pw0;
@@ -301,7 +315,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_score_i8(DevicePlan plan, const 
         ncol = __builtin_amdgcn_readfirstlane(ncol);
         nclass = __builtin_amdgcn_readfirstlane(nclass);
         tb = __builtin_amdgcn_readfirstlane(tb);
-        int *sbuf = stagebuf[wave];
+        int *sbuf = (kStageInTile ? (int *)&tile[0][0] : stage_own) + wave * (64 * 16);
         auto flush_tile = [&](auto jc) __attribute__((always_inline)) {
             constexpr int j = decltype(jc)::value;
             __builtin_amdgcn_sched_barrier(0);
@@ -345,89 +359,139 @@ __global__ __launch_bounds__(64 * WAVES) void k_score_i8(DevicePlan plan, const 
         [&]<int... J>(std::integer_sequence<int, J...>) __attribute__((always_inline)) {
             (flush_tile(std::integral_constant<int, J>{}), ...);
         }(std::make_integer_sequence<int, TMAX>{});
+        // staging inside the .bed tiles: no wave may store the next batch's
+        // tiles while another still reads its staged sums
+        if constexpr (kStageInTile) lds_barrier_i8();
     };
 
-    auto compute = [&](int buf) __attribute__((always_inline)) {
+    // compute(nwc, par, buf): one stage.  With a compile-time layout
+    // (nwc >= 0: the first nwc tiles are weight tiles, the others count
+    // tiles) the stage is straight-line code, so the scheduler can overlap
+    // one stage's decode with the previous stage's MFMAs; tile slots past
+    // ntile are computed on a duplicated fragment and never flushed.  With
+    // nwc < 0 the layout is read at run time (per-tile branches).
+    auto compute = [&](Acc &acc, auto nwc, uint32_t par, int buf) __attribute__((always_inline)) {
+        constexpr int NW = decltype(nwc)::value;
         const uint32_t *t = tile[buf];
         const u32x4_t X = *(const u32x4_t *)&t[tile_dword(wave * 16 + li, 4 * lg)];
         const u32x4_t F = *(const u32x4_t *)&t[tile_dword(kBytes, 4 * lg)];
         u32x4_t bv[TMAX];
 #pragma unroll
-        for (int j = 0; j < TMAX; ++j) bv[j] = btile[buf][j * 64 + lane];
+        for (int j = 0; j < TMAX; ++j) bv[j] = btile[par][buf][j * 64 + lane];
         uint32_t sel[4][4];   // [person t][dword q]
 #pragma unroll
         for (int q = 0; q < 4; ++q)
 #pragma unroll
             for (int tt = 0; tt < 4; ++tt) sel[tt][q] = ((X[q] >> (2 * tt)) & 0x03030303u) | F[q];
-        i32x4_t A[4];
+        auto mfma_tile = [&](int j, const i32x4_t (&op)[4]) __attribute__((always_inline)) {
+            const i32x4_t B = i32x4_t{(int)bv[j][0], (int)bv[j][1], (int)bv[j][2], (int)bv[j][3]};
 #pragma unroll
-        for (int tt = 0; tt < 4; ++tt)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) A[tt][q] = (int)dose_i8(sel[tt][q]);
-#pragma unroll
-        for (int j = 0; j < TMAX; ++j) {
-            if ((uint32_t)j < nw) {
-                const i32x4_t B = i32x4_t{(int)bv[j][0], (int)bv[j][1], (int)bv[j][2], (int)bv[j][3]};
-#pragma unroll
-                for (int tt = 0; tt < 4; ++tt)
-                    acc[tt][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[tt], B, acc[tt][j], 0, 0, 0);
-            }
-        }
-        if (nw < ntile) {
-            i32x4_t M[4];
+            for (int tt = 0; tt < 4; ++tt)
+                acc[tt][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(op[tt], B, acc[tt][j], 0, 0, 0);
+        };
+        i32x4_t A[4], M[4];
+        if constexpr (NW != 0) {
 #pragma unroll
             for (int tt = 0; tt < 4; ++tt)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) M[tt][q] = (int)miss_i8(sel[tt][q]);
+                for (int q = 0; q < 4; ++q) A[tt][q] = (int)dose_i8(sel[tt][q]);
+        }
+        if constexpr (NW >= 0) {
+            if constexpr (NW < TMAX) {
 #pragma unroll
-            for (int j = 0; j < TMAX; ++j) {
-                if ((uint32_t)j >= nw && (uint32_t)j < ntile) {
-                    const i32x4_t B = i32x4_t{(int)bv[j][0], (int)bv[j][1], (int)bv[j][2], (int)bv[j][3]};
+                for (int tt = 0; tt < 4; ++tt)
 #pragma unroll
-                    for (int tt = 0; tt < 4; ++tt)
-                        acc[tt][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(M[tt], B, acc[tt][j], 0, 0, 0);
-                }
+                    for (int q = 0; q < 4; ++q) M[tt][q] = (int)miss_i8(sel[tt][q]);
+            }
+#pragma unroll
+            for (int j = 0; j < TMAX; ++j) mfma_tile(j, j < NW ? A : M);
+        } else {
+#pragma unroll
+            for (int j = 0; j < TMAX; ++j)
+                if ((uint32_t)j < nw) mfma_tile(j, A);
+            if (nw < ntile) {
+#pragma unroll
+                for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) M[tt][q] = (int)miss_i8(sel[tt][q]);
+#pragma unroll
+                for (int j = 0; j < TMAX; ++j)
+                    if ((uint32_t)j >= nw && (uint32_t)j < ntile) mfma_tile(j, M);
             }
         }
     };
 
     if (r_begin >= r_end) {
-        flush(0, true, true);
+        Acc acc;
+        zero_acc(acc);
+        flush(acc, 0, true, true);
         return;
     }
-    // Stages are processed in batches of D: the loads of all D stages are
-    // issued together at the top of a batch and consumed stage by stage.  No
-    // load result is carried across loop iterations (a value loaded in one
-    // iteration and consumed in the next gets copied by the register
-    // allocator, and the copy waits for every load in flight).
+    // Stages are processed in batches of D, software-pipelined: a batch's
+    // row loads and B-fragment DMA are issued before the previous batch is
+    // computed, so their latency overlaps the MFMAs; a batch is stored to
+    // LDS only once its loads have landed.
     const uint64_t st0 = (r_begin / (D * kI8Stage)) * (D * kI8Stage);
     const uint64_t nst = (r_end - st0 + kI8Stage - 1) / kI8Stage;
-    // Outer loop over scale blocks, inner loop over batches of the block:
-    // the flush sits after the inner loop on every path, so the
-    // accumulators carried by the inner loop are only ever MFMA results and
-    // stay in accumulation registers.
-    bool first = true;
-    for (uint64_t i0 = 0; i0 < nst;) {
-        const uint64_t blk = (st0 + i0 * kI8Stage) / kI8FlushRows;
-        const uint64_t blk_end = ((blk + 1) * kI8FlushRows - st0) / kI8Stage;   // a multiple of D
-        const uint64_t i_end = blk_end < nst ? blk_end : nst;
-        do {   // i0 < i_end on entry: the block holds stage i0
-            [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
-                (issue(I, st0 + (i0 + I) * kI8Stage), ...);
-            }(std::make_integer_sequence<int, D>{});
-            [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
-                ((i0 + I < nst ? (store(I, I), 0) : 0), ...);
-            }(std::make_integer_sequence<int, D>{});
-            dma_lds_barrier();   // B fragments landed and every tile stored
-            [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
-                ((i0 + I < nst ? (compute(I), 0) : 0), ...);
-            }(std::make_integer_sequence<int, D>{});
-            lds_barrier_i8();   // every wave is done with the tiles before the next batch stores
-            i0 += D;
-        } while (i0 < i_end);
-        flush(blk, first, false);
-        first = false;
+    auto issue_batch = [&](uint32_t par, uint64_t i) __attribute__((always_inline)) {
+        // stages past the split are loaded clamped and never stored
+        [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
+            (issue(par, I, st0 + (i + I) * kI8Stage), ...);
+        }(std::make_integer_sequence<int, D>{});
+    };
+    auto fetch_batch = [&](uint32_t par, uint64_t i) __attribute__((always_inline)) {
+        [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
+            (fetch_b(par, I, (st0 + (i + I) * kI8Stage) < last_st ? st0 + (i + I) * kI8Stage : last_st), ...);
+        }(std::make_integer_sequence<int, D>{});
+    };
+    auto main_loop = [&](auto nwc) __attribute__((always_inline)) {
+        Acc acc;   // one accumulator set per copy of the loop
+        zero_acc(acc);
+        uint32_t par = 0;
+        uint64_t i0 = 0;
+        issue_batch(par, i0);
+        fetch_batch(par, i0);
+        // Outer loop over scale blocks, inner loop over batches of the block:
+        // the flush sits after the inner loop on every path, so the
+        // accumulators carried by the inner loop are only ever MFMA results and
+        // stay in accumulation registers.
+        bool first = true;
+        do {   // nst > 0
+            const uint64_t blk = (st0 + i0 * kI8Stage) / kI8FlushRows;
+            const uint64_t blk_end = ((blk + 1) * kI8FlushRows - st0) / kI8Stage;   // a multiple of D
+            const uint64_t i_end = blk_end < nst ? blk_end : nst;
+            do {   // i0 < i_end on entry: the block holds stage i0
+                [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
+                    ((i0 + I < nst ? (store(I, I), 0) : 0), ...);
+                }(std::make_integer_sequence<int, D>{});
+                dma_lds_barrier();   // this batch's B fragments landed and its tiles are stored
+                issue_batch(par ^ 1u, i0 + D);
+                [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
+                    ((i0 + I < nst ? (compute(acc, nwc, par, I), 0) : 0), ...);
+                }(std::make_integer_sequence<int, D>{});
+                // after this wave's last read of the batch's B fragments: the
+                // DMA into the other half never overlaps an LDS read in flight
+                fetch_batch(par ^ 1u, i0 + D);
+                lds_barrier_i8();   // every wave is done with the tiles before the next batch stores
+                par ^= 1u;
+                i0 += D;
+            } while (i0 < i_end);
+            flush(acc, blk, first, false);
+            first = false;
+        } while (i0 < nst);
+    };
+    // One copy of the loop per group layout on the common path (dword row
+    // loads, identity rows): nw is fixed per workgroup, so the choice is made
+    // once, and no accumulator value is merged across the copies.
+    if constexpr (VEC && IDENT) {
+        [&]<int... N>(std::integer_sequence<int, N...>) __attribute__((always_inline)) {
+            ((nw == (uint32_t)N ? (main_loop(std::integral_constant<int, N>{}), 0) : 0), ...);
+        }(std::make_integer_sequence<int, TMAX + 1>{});
+    } else {
+        main_loop(std::integral_constant<int, -1>{});
     }
+    // the last, unused batch's DMA must land before the workgroup's LDS is released
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
 // Rough cost of one group pass per wave and stage, in cycles: decode VALU
