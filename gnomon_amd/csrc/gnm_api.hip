@@ -524,7 +524,7 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
     if ((st = c->alloc(&c->d_counts, cells)) != GNM_OK) return st;
     GNM_HIP(launch_fill_f64(c->d_scores, cells, 0.0, c->stream));
     GNM_HIP(launch_fill_u32(c->d_counts, cells, 0u, c->stream));
-    c->splits = mfma_weights ? mfma_splits(c->N, tiles) : fp32_splits(c->N, K);
+    c->splits = mfma_weights ? (int8 ? i8_splits(c->N, tiles) : mfma_splits(c->N, tiles)) : fp32_splits(c->N, K);
     if (!mfma_weights && !c->fp32_count_in_kernel)
         c->splits = std::max(c->splits, mfma_splits(c->N, tiles));
     const uint64_t ncol_part = mfma_weights ? (uint64_t)L * K : K;
@@ -965,8 +965,9 @@ gnm_status gnm_dense_forward_device(const uint8_t *d_rows, uint64_t bytes_per_va
         const bool int8 = is_int8(precision);
         const uint64_t Rpad = std::max<uint64_t>((V + 63) / 64 * 64, 64);
         const uint32_t Lw = L == 1 ? 0 : L;
+        const uint32_t ntl = (uint32_t)((Lw * K + 15) / 16 + 1);
         const uint32_t S = precision == GNM_PRECISION_FP32 ? fp32_splits(N, K)
-                                                           : mfma_splits(N, (uint32_t)((Lw * K + 15) / 16 + 1));
+                                                           : (int8 ? i8_splits(N, ntl) : mfma_splits(N, ntl));
         gnm_status s;
         DevicePlan pl{};
         uint32_t *prog_var, *flipmask, *prog_cls, *score_class, *cnt_tmp, *cpart;

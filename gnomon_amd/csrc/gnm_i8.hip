@@ -501,6 +501,27 @@ static uint64_t i8_pass_cost(uint32_t ntiles, uint32_t tmax) {
     return groups * (240u + 64u * tmax);
 }
 
+// default tiles per pass: the cheapest of 3, 4, 6 and 8 by the cost model
+uint32_t i8_default_tiles_per_pass(uint32_t ntiles) {
+    uint32_t tmax = 3;
+    for (uint32_t c : {4u, 6u, 8u})
+        if (i8_pass_cost(ntiles, c) < i8_pass_cost(ntiles, tmax)) tmax = c;
+    return tmax;
+}
+
+// Variant splits of the int8 kernel: enough that all tile groups together
+// launch about 12k workgroups (~48 per CU, so that three stay resident per
+// CU through the tail and many row strips are read at once), a multiple of
+// 8 (whole splits per XCD), at most 64 (the FP64 partials grow with it).
+uint32_t i8_splits(uint64_t num_people, uint32_t ntiles) {
+    const uint32_t tmax = i8_default_tiles_per_pass(ntiles ? ntiles : 1);
+    const uint64_t groups = ((ntiles ? ntiles : 1) + tmax - 1) / tmax;
+    const uint64_t per = ((num_people + 255) / 256) * groups;
+    uint64_t s = (12288 + per - 1) / per;
+    s = (s + 7) / 8 * 8;
+    return (uint32_t)(s < 8 ? 8 : (s > 64 ? 64 : s));
+}
+
 hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t stride, uint64_t bpv, uint64_t v0,
                            uint64_t p0, uint64_t p1, uint64_t num_people, uint32_t splits, uint32_t tile_begin,
                            uint32_t tile_end, bool identity, double *part, uint32_t *cpart, hipStream_t stream,
@@ -515,12 +536,7 @@ hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t
     const uint32_t ntiles = tile_end - tile_begin;
     // tiles per workgroup pass: every group computes tmax tile slots, so the
     // default takes the cheapest of 3, 4, 6 and 8 by the cost model above
-    uint32_t tmax = tiles_per_pass;
-    if (!tmax) {
-        tmax = 3;
-        for (uint32_t c : {4u, 6u, 8u})
-            if (i8_pass_cost(ntiles, c) < i8_pass_cost(ntiles, tmax)) tmax = c;
-    }
+    const uint32_t tmax = tiles_per_pass ? tiles_per_pass : i8_default_tiles_per_pass(ntiles);
     if (tmax != 3 && tmax != 4 && tmax != 6 && tmax != 8) return hipErrorInvalidValue;
     const uint32_t depth = batch_stages ? batch_stages : 4u;
     if (depth != 2 && depth != 4) return hipErrorInvalidValue;

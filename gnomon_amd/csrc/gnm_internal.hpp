@@ -82,6 +82,8 @@ hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t
                            uint64_t p0, uint64_t p1, uint64_t num_people, uint32_t splits, uint32_t tile_begin,
                            uint32_t tile_end, bool identity, double *part, uint32_t *cpart, hipStream_t stream,
                            uint32_t tiles_per_pass = 0, uint32_t batch_stages = 0);
+// variant splits of the int8 kernel for num_people and ntiles tiles
+uint32_t i8_splits(uint64_t num_people, uint32_t ntiles);
 hipError_t launch_forward_frag_i8(const float *W, uint64_t V, uint64_t K, uint32_t L, const int *exps,
                                   uint32_t wtiles, uint32_t tiles, uint64_t nks, uint32_t *frag, double *scales,
                                   hipStream_t stream);
