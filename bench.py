@@ -201,7 +201,8 @@ def run_differentiation(w, args):
     """Ridge-penalised regression through the differentiable path at 50,000
     people x 500,000 variants x 16 scores: y = G w* + N(0, 1),
     L(W) = sum_k mean_p (S[p,k] - y[p])^2 + lam ||W||^2 with S = G^T W from the
-    split-BF16 forward and dL/dW from the adjoint kernel.  Times 20 gradient
+    forward of --precision (split-int8 by default) and dL/dW from the adjoint
+    kernel of the same arithmetic.  Times 20 gradient
     steps; checks the gradient against central finite differences on 100
     random weights and against the FP64 CPU gradient of those weights."""
     import numpy as np
@@ -212,6 +213,7 @@ def run_differentiation(w, args):
     from oracle import oracle as orc
 
     N, V, K, lam = w.num_people, w.num_variants, w.num_scores, 1e-3
+    P = args.precision
     bpv = w.bytes_per_variant
     stride = (bpv + 255) // 256 * 256
     rng = np.random.default_rng(w.seed + 7)
@@ -231,7 +233,7 @@ def run_differentiation(w, args):
 
     # gradient at W0: autograd, FP64 CPU, central FD
     W = torch.from_numpy(W0).cuda().requires_grad_(True)
-    loss = loss_fn(ag.polygenic_score(W, d_bed, d_flip, N, "bf16x3"), W)
+    loss = loss_fn(ag.polygenic_score(W, d_bed, d_flip, N, P), W)
     loss.backward()
     g_auto = W.grad.detach().cpu().numpy().astype(np.float64)
     S_ref, _ = ag.dense_forward(torch.from_numpy(W0).cuda(), d_bed, d_flip, N, "fp32")
@@ -267,20 +269,20 @@ def run_differentiation(w, args):
     lam_max = 0.0
     for _ in range(12):
         s, _ = ag.dense_forward(u, d_bed, d_flip, N, "fp32")
-        hu = ag.dense_adjoint((2.0 / N) * s, d_bed, d_flip, V, "bf16x3").double() + 2 * lam * u.double()
+        hu = ag.dense_adjoint((2.0 / N) * s, d_bed, d_flip, V, P).double() + 2 * lam * u.double()
         lam_max = float(hu.norm() / u.double().norm())
         u = (hu / hu.norm()).float()
     Wt = torch.zeros(V, K, dtype=torch.float32, device="cuda", requires_grad=True)
     opt = torch.optim.SGD([Wt], lr=1.0 / lam_max)
     for _ in range(2):    # warm-up steps on a throw-away copy
         Wc = Wt.detach().clone().requires_grad_(True)
-        loss_fn(ag.polygenic_score(Wc, d_bed, d_flip, N, "bf16x3"), Wc).backward()
+        loss_fn(ag.polygenic_score(Wc, d_bed, d_flip, N, P), Wc).backward()
     torch.cuda.synchronize()
     losses = []
     t0 = time.perf_counter()
     for _ in range(20):
         opt.zero_grad()
-        loss = loss_fn(ag.polygenic_score(Wt, d_bed, d_flip, N, "bf16x3"), Wt)
+        loss = loss_fn(ag.polygenic_score(Wt, d_bed, d_flip, N, P), Wt)
         loss.backward()
         opt.step()
         losses.append(loss.detach())
@@ -291,9 +293,9 @@ def run_differentiation(w, args):
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     R = torch.randn(N, K, device="cuda", dtype=torch.float32)
     ev[0].record()
-    ag.dense_forward(Wt.detach(), d_bed, d_flip, N, "bf16x3")
+    ag.dense_forward(Wt.detach(), d_bed, d_flip, N, P)
     ev[1].record()
-    ag.dense_adjoint(R, d_bed, d_flip, V, "bf16x3")
+    ag.dense_adjoint(R, d_bed, d_flip, V, P)
     ev[2].record()
     torch.cuda.synchronize()
     fwd_ms, adj_ms = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
@@ -303,7 +305,7 @@ def run_differentiation(w, args):
     out = {
         "metric": "gradient_steps/s", "value": 1e3 / ms_step, "unit": "steps/s", "n_gpus": 1, "steps": 20,
         "warmup": 2, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16x3->f32->f64", "data": "synthetic (device-generated HWE genotypes)",
+        "dtype": DTYPE[P], "adjoint_precision": ag._adjoint_precision(P, d_bed), "data": "synthetic (device-generated HWE genotypes)",
         "config": {"workload": w.name, "people": N, "variants": V, "scores": K, "lambda": lam,
                    "step": "SGD, lr = 1/lambda_max"},
         "forward_ms": fwd_ms, "adjoint_ms": adj_ms,

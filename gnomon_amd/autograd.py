@@ -63,8 +63,20 @@ def dense_adjoint(R, bed, flip, num_variants, precision="bf16x3"):
     dW = torch.empty(num_variants, K, dtype=torch.float32, device=R.device)
     _lib.check(_lib.lib().gnm_dense_adjoint_device(
         _ptr(bed), bed.shape[1], num_variants, N, _ptr(flip), _ptr(R), K,
-        _PREC["bf16x2" if precision == "bf16x2" else "bf16x3"], _ptr(dW), _stream()))
+        _PREC[_adjoint_precision(precision, bed)], _ptr(dW), _stream()))
     return dW
+
+
+def _adjoint_precision(precision, bed):
+    """The adjoint's arithmetic for a forward precision: split-int8 for the
+    int8 paths when the rows are 4-byte aligned with a row stride that is a
+    multiple of 4 (what the int8 kernel's dword loads need), else split-BF16
+    (bf16x3 unless bf16x2 is asked for)."""
+    if precision in ("i8x4", "i8x3"):
+        if bed.data_ptr() % 4 == 0 and bed.shape[1] % 4 == 0:
+            return precision
+        return "bf16x3"
+    return "bf16x2" if precision == "bf16x2" else "bf16x3"
 
 
 class PolygenicScore(torch.autograd.Function):
@@ -74,7 +86,8 @@ class PolygenicScore(torch.autograd.Function):
     (packed SNP-major rows); flip: uint8 CUDA tensor [V] (1 = effect allele
     is A1) or None.  Returns S [N, K] float64.  precision selects the forward
     arithmetic ("bf16x3", "bf16x2", "i8x4", "i8x3" or "fp32"); the adjoint
-    runs split-BF16 (bf16x3 unless "bf16x2" is asked for).
+    runs split-int8 with the int8 forwards, else split-BF16 (bf16x3 unless
+    "bf16x2" is asked for).
     """
 
     @staticmethod

@@ -1051,9 +1051,43 @@ gnm_status gnm_dense_adjoint_device(const uint8_t *d_rows, uint64_t bytes_per_va
         if (V && (!d_rows || !d_grad_weights)) return fail(GNM_ERR_INVALID_ARG, "NULL device pointer");
         if (!d_grad_scores) return fail(GNM_ERR_INVALID_ARG, "NULL grad_scores");
         if (bytes_per_variant < (N + 3) / 4) return fail(GNM_ERR_INVALID_ARG, "bytes_per_variant too small");
-        if (precision == GNM_PRECISION_AUTO || is_int8(precision)) precision = GNM_PRECISION_BF16X3;
+        if (precision == GNM_PRECISION_AUTO) precision = GNM_PRECISION_BF16X3;
+        if (is_int8(precision)) {
+            // split-int8 adjoint: R limbs per block of kI8FlushRows people
+            const uint32_t L = limbs_of(precision);
+            const uint64_t Kp = (K + 15) / 16 * 16;
+            const uint32_t used = (uint32_t)(L * (Kp / 16));
+            const uint32_t tiles = (used + 3) / 4 * 4;
+            if (tiles > 16) return fail(GNM_ERR_UNSUPPORTED, "the int8 adjoint supports up to 64 scores per call");
+            if (((reinterpret_cast<uintptr_t>(d_rows) | bytes_per_variant) & 3u) || bytes_per_variant >= (1ull << 25))
+                return fail(GNM_ERR_UNSUPPORTED,
+                            "the int8 adjoint needs 4-byte aligned rows with a stride that is a multiple of 4 "
+                            "(below 2^25)");
+            hipStream_t st = (hipStream_t)stream_;
+            DenseWorkspace *ws = workspace_for_current_device();
+            if (!ws) return fail(GNM_ERR_NO_DEVICE, "no current HIP device");
+            std::lock_guard<std::mutex> lk(ws->mu);
+            const uint64_t nss = ((N + 3) / 4 + 63) / 64;       // super-steps of 256 people
+            const uint64_t nks = nss * 4;
+            const uint64_t nblk = (N + kI8FlushRows - 1) / kI8FlushRows;
+            gnm_status s;
+            int *exps;
+            uint32_t *frag;
+            double *scales, *acc;
+            if ((s = ws_typed(ws, 11, nblk * K, st, &exps)) != GNM_OK) return s;
+            if ((s = ws_typed(ws, 12, nks * tiles * 64 * 4, st, &frag)) != GNM_OK) return s;
+            if ((s = ws_typed(ws, 13, nblk * L * Kp, st, &scales)) != GNM_OK) return s;
+            if ((s = ws_typed(ws, 15, std::max<uint64_t>(V, 1) * Kp, st, &acc)) != GNM_OK) return s;
+            GNM_HIP(launch_fill_f64(scales, nblk * L * Kp, 0.0, st));
+            GNM_HIP(launch_block_exponent(d_grad_scores, N, K, K, kI8FlushRows, L, true, exps, st));
+            GNM_HIP(launch_adjoint_frag_i8(d_grad_scores, N, K, Kp, L, exps, tiles, used, nks, frag, scales, st));
+            GNM_HIP(launch_adjoint_i8(d_rows, bytes_per_variant, V, N, d_flip, frag, scales, tiles, K, Kp, L, acc,
+                                      d_grad_weights, st));
+            return GNM_OK;
+        }
         if (precision != GNM_PRECISION_BF16X2 && precision != GNM_PRECISION_BF16X3)
-            return fail(GNM_ERR_UNSUPPORTED, "the adjoint runs in split-BF16 (bf16x2 or bf16x3)");
+            return fail(GNM_ERR_UNSUPPORTED, "the adjoint runs in split-BF16 (bf16x2, bf16x3) or split-int8 "
+                                             "(i8x3, i8x4)");
         const uint32_t L = precision == GNM_PRECISION_BF16X2 ? 2 : 3;
         const uint64_t Kp = (K + 15) / 16 * 16;
         const uint32_t tiles = (uint32_t)(L * (Kp / 16));

@@ -642,6 +642,247 @@ hipError_t launch_forward_frag_i8(const float *W, uint64_t V, uint64_t K, uint32
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Split-int8 adjoint dW = G^T R.  The reduction runs over people, which are
+// contiguous inside a .bed row, so the MFMA A operand (16 variants x 64
+// people) comes straight from each lane's own row: no transpose.  People are
+// taken in super-steps of 256 (64 packed bytes per row = 4 k-steps); lane
+// (i, g) loads bytes 64S + 16g .. +15 of its row with one 16-byte load, and
+// dword d of that load is its part of k-step 4S + d.  In that k-step, A
+// dword t byte b (K position 4t + b of the lane's 16) is person
+// 256S + 64g + 16d + 4b + t; the B fragments of R follow the same order.
+// R is split per (block of kI8FlushRows people, column) into L signed int8
+// limbs like the forward weights; the int32 block sums are flushed, scaled,
+// into an FP64 workspace [V][Kp].
+// ---------------------------------------------------------------------------
+
+// B fragments [ks][tile][lane][16 i8] of R [N][K]: column c = limb*Kp + k;
+// tiles past L*Kp/16 and people past N are zero.  One thread per
+// (k-step, tile, lane).
+__global__ void k_adjoint_frag_i8(const float *__restrict__ R, uint64_t N, uint64_t K, uint64_t Kp, uint32_t L,
+                                  const int *__restrict__ exps, uint32_t tiles, uint32_t tiles_used, uint64_t nks,
+                                  uint32_t *__restrict__ frag, double *__restrict__ scales) {
+    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= nks * tiles * 64) return;
+    const uint32_t lane = id & 63;
+    const uint32_t tile = (uint32_t)((id >> 6) % tiles);
+    const uint64_t ks = (id >> 6) / tiles;
+    const uint32_t n = lane & 15, g = lane >> 4;
+    const uint64_t c = (uint64_t)tile * 16 + n;
+    const uint32_t limb = (uint32_t)(c / Kp);
+    const uint64_t k = c % Kp;
+    const bool col_ok = tile < tiles_used && k < K;
+    const uint64_t pbase = (ks >> 2) * 256 + 64 * g + 16 * (ks & 3);
+    uint32_t out[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+        const int t = p >> 2, b = p & 3;
+        const uint64_t person = pbase + 4 * b + t;
+        int32_t val = 0;
+        if (col_ok && person < N) {
+            const uint64_t blk = person / kI8FlushRows;
+            val = limb_i8(R[person * K + k], exps[blk * K + k], L, limb);
+        }
+        out[p >> 2] |= ((uint32_t)val & 0xFFu) << (8 * (p & 3));
+    }
+    uint32_t *dst = frag + id * 4;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dst[q] = out[q];
+    if (col_ok && g == 0 && (ks * kI8Stage) % kI8FlushRows == 0) {
+        const uint64_t blk = ks * kI8Stage / kI8FlushRows;
+        scales[blk * (L * Kp) + c] = ldexp(1.0, exps[blk * K + k] + 8 * (int)(L - 1 - limb));
+    }
+}
+
+hipError_t launch_adjoint_frag_i8(const float *R, uint64_t N, uint64_t K, uint64_t Kp, uint32_t L, const int *exps,
+                                  uint32_t tiles, uint32_t tiles_used, uint64_t nks, uint32_t *frag, double *scales,
+                                  hipStream_t stream) {
+    const uint64_t n = nks * tiles * 64;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adjoint_frag_i8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, R, N, K, Kp, L,
+                       exps, tiles, tiles_used, nks, frag, scales);
+    return hipGetLastError();
+}
+
+// One workgroup = 4 waves x MT 16-variant tiles.  Per super-step the
+// workgroup's 256 threads load the B fragments of 4 k-steps (16 B x TILES
+// each) and store them to LDS (double-buffered, one barrier per
+// super-step); each lane loads its MT rows' 16 bytes.  Both are loaded one
+// super-step ahead into registers.
+template <int TILES, int MT>
+__global__ __launch_bounds__(256) void k_adjoint_i8(const uint8_t *__restrict__ rows, uint64_t stride, uint64_t V,
+                                                    uint64_t N, const uint8_t *__restrict__ flip,
+                                                    const u32x4_t *__restrict__ frag,
+                                                    const double *__restrict__ scales, uint64_t Kp, uint32_t L,
+                                                    double *__restrict__ ws) {
+    constexpr int kBPer = TILES;                  // B entries (16 B) per thread per super-step
+    __shared__ u32x4_t bt[2][4 * TILES * 64];     // [buf][(k-step d, tile j, lane)]
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int li = lane & 15, lg = lane >> 4;
+    const uint64_t wv0 = ((uint64_t)blockIdx.x * 4 + wave) * (16 * MT);   // first variant of the wave
+    const uint64_t bpv = (N + 3) / 4;
+    const uint64_t nss = (bpv + 63) / 64;                                // super-steps
+    const uint32_t tpl = (uint32_t)(Kp / 16);                            // tiles per limb
+    // rows of the wave: one descriptor, lane offsets (16mt + i) * stride + 16g;
+    // rows past V read as zero (range check), bytes past the row end belong
+    // to people whose R limbs are zero
+    const uint64_t rem = wv0 < V ? (V - wv0) * stride : 0;
+    const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(rows + (wv0 < V ? wv0 : 0) * stride), 0, (int)(rem < 0x7FFFFFFFull ? rem : 0x7FFFFFFFull),
+        kI8RsrcWord3);
+    uint32_t voff[MT], F[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+        const uint64_t v = wv0 + 16 * mt + li;
+        voff[mt] = (uint32_t)((16 * mt + li) * stride + 16 * lg);
+        F[mt] = (v < V && flip && flip[v]) ? 0x04040404u : 0u;
+    }
+    const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)frag, 0, 0x7FFFFFFF, kI8RsrcWord3);
+    auto load_ss = [&](uint64_t ss, u32x4_t (&x)[MT], u32x4_t (&b)[kBPer]) __attribute__((always_inline)) {
+        const uint32_t so = (uint32_t)(64 * ss);
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+            x[mt] = __builtin_bit_cast(u32x4_t, __builtin_amdgcn_raw_buffer_load_b128(rr, voff[mt], so, 0));
+        const uint64_t e0 = ss * (4 * TILES * 64);   // first B entry of the super-step
+#pragma unroll
+        for (int m = 0; m < kBPer; ++m)
+            b[m] = __builtin_bit_cast(u32x4_t, __builtin_amdgcn_raw_buffer_load_b128(
+                                                   rb, (uint32_t)((e0 + tid + 256 * m) * 16), 0, 0));
+    };
+    i32x4_t acc[MT][TILES];
+    auto zero = [&]() __attribute__((always_inline)) {
+        int z = 0;
+        asm volatile("" : "+v"(z));
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int j = 0; j < TILES; ++j) acc[mt][j] = i32x4_t{z, z, z, z};
+    };
+    zero();
+    // block flush: acc[mt][j][q] is row 4g + q of tile mt (variant
+    // wv0 + 16mt + 4g + q), column j*16 + i; limbs of score k = jj*16 + i sit
+    // in tiles l*tpl + jj of the same lane
+    auto flush = [&](uint64_t blk, bool first) __attribute__((always_inline)) {
+        // opaque copies: what the flush derives from them is formed here,
+        // not hoisted out of the loop into live registers
+        uint32_t kp = (uint32_t)Kp, tp = tpl, ll = L, lane_i = (uint32_t)li;
+        uint64_t v0 = wv0 + 4 * (uint64_t)lg;
+        asm volatile("" : "+v"(kp), "+v"(tp), "+v"(ll), "+v"(lane_i), "+v"(v0));
+        kp = __builtin_amdgcn_readfirstlane(kp);
+        tp = __builtin_amdgcn_readfirstlane(tp);
+        ll = __builtin_amdgcn_readfirstlane(ll);
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            // one tile row of sums at a time: kept in accumulation registers
+            // up to here, so only TILES x 4 of them are converted at once
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < TILES; ++j) asm volatile("" : "+a"(acc[mt][j]));
+#pragma unroll 1
+            for (uint32_t jj = 0; jj < tp; ++jj) {
+                double sc[TILES];
+#pragma unroll
+                for (int j = 0; j < TILES; ++j) {
+                    const uint32_t l = (uint32_t)j / tp;
+                    sc[j] = ((uint32_t)j % tp == jj && l < ll)
+                                ? scales[blk * (ll * (uint64_t)kp) + (uint64_t)j * 16 + lane_i] : 0.0;
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint64_t v = v0 + 16 * mt + q;
+                    double sum = 0.0;
+#pragma unroll
+                    for (int j = 0; j < TILES; ++j) sum += (double)acc[mt][j][q] * sc[j];
+                    if (v < V) {
+                        double *d = ws + v * kp + (uint64_t)jj * 16 + lane_i;
+                        *d = first ? sum : *d + sum;
+                    }
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        zero();
+    };
+
+    u32x4_t xa[MT], ba[kBPer];
+    load_ss(0, xa, ba);
+    // outer loop over person blocks, inner loop over the block's
+    // super-steps: the flush sits after the inner loop on every path, so the
+    // accumulators stay in accumulation registers
+    constexpr uint64_t kSSPerBlk = kI8FlushRows / 256;
+    uint64_t ss = 0;
+    bool first = true;
+    do {   // nss > 0
+        const uint64_t blk = ss / kSSPerBlk;
+        const uint64_t ss_end = (blk + 1) * kSSPerBlk < nss ? (blk + 1) * kSSPerBlk : nss;
+        do {
+            const uint32_t buf = (uint32_t)(ss & 1);
+            // this super-step's B fragments into LDS (the other half may still
+            // be read by a wave finishing the previous super-step)
+#pragma unroll
+            for (int m = 0; m < kBPer; ++m) bt[buf][tid + 256 * m] = ba[m];
+            u32x4_t xc[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) xc[mt] = xa[mt];
+            lds_barrier_i8();
+            // next super-step in flight during this one's MFMAs (clamped at
+            // the end: the extra loads are never used)
+            load_ss(ss + 1 < nss ? ss + 1 : ss, xa, ba);
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                u32x4_t bv[TILES];
+#pragma unroll
+                for (int j = 0; j < TILES; ++j) bv[j] = bt[buf][(d * TILES + j) * 64 + lane];
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) {
+                    i32x4_t A;
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) A[t] = (int)dose_i8(((xc[mt][d] >> (2 * t)) & 0x03030303u) | F[mt]);
+#pragma unroll
+                    for (int j = 0; j < TILES; ++j) {
+                        const i32x4_t B = i32x4_t{(int)bv[j][0], (int)bv[j][1], (int)bv[j][2], (int)bv[j][3]};
+                        acc[mt][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, B, acc[mt][j], 0, 0, 0);
+                    }
+                }
+            }
+            ++ss;
+        } while (ss < ss_end);
+        flush(blk, first);
+        first = false;
+    } while (ss < nss);
+}
+
+// dW [V][K] f32 from the FP64 workspace [V][Kp]
+__global__ void k_adjoint_out(const double *__restrict__ ws, uint64_t V, uint64_t K, uint64_t Kp,
+                              float *__restrict__ dW) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < V * K) dW[i] = (float)ws[(i / K) * Kp + i % K];
+}
+
+hipError_t launch_adjoint_i8(const uint8_t *rows, uint64_t stride, uint64_t V, uint64_t N, const uint8_t *flip,
+                             const uint32_t *frag, const double *scales, uint32_t tiles, uint64_t K, uint64_t Kp,
+                             uint32_t L, double *ws, float *dW, hipStream_t stream) {
+    if (V == 0) return hipSuccess;
+    if ((reinterpret_cast<uintptr_t>(rows) | stride) & 3u) return hipErrorInvalidValue;
+    if (stride >= (1ull << 25)) return hipErrorInvalidValue;
+#define GNM_ADJ(T, M)                                                                                              \
+    hipLaunchKernelGGL((k_adjoint_i8<T, M>), dim3((unsigned)((V + 64 * M - 1) / (64 * M))), dim3(256), 0, stream, \
+                       rows, stride, V, N, flip, (const u32x4_t *)frag, scales, Kp, L, ws)
+    if (tiles == 4) GNM_ADJ(4, 4);
+    else if (tiles == 8) GNM_ADJ(8, 2);
+    else if (tiles == 12) GNM_ADJ(12, 1);
+    else if (tiles == 16) GNM_ADJ(16, 1);
+    else return hipErrorInvalidValue;
+#undef GNM_ADJ
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const uint64_t n = V * K;
+    hipLaunchKernelGGL(k_adjoint_out, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, ws, V, K, Kp, dW);
+    return hipGetLastError();
+}
+
 // per-row selector bytes (4 = flipped row) from a flip byte array
 __global__ void k_flip_bytes(const uint8_t *__restrict__ flip, uint64_t V, uint64_t Rpad, uint8_t *__restrict__ out) {
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;

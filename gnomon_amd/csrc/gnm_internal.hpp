@@ -87,6 +87,15 @@ uint32_t i8_splits(uint64_t num_people, uint32_t ntiles);
 hipError_t launch_forward_frag_i8(const float *W, uint64_t V, uint64_t K, uint32_t L, const int *exps,
                                   uint32_t wtiles, uint32_t tiles, uint64_t nks, uint32_t *frag, double *scales,
                                   hipStream_t stream);
+// split-int8 adjoint (gnm_i8.hip): R fragments in super-step people order,
+// then dW = G^T R through an FP64 workspace [V][Kp]; tiles = L*Kp/16 padded
+// to 4, 8, 12 or 16; rows 4-byte aligned, stride a multiple of 4 below 2^25
+hipError_t launch_adjoint_frag_i8(const float *R, uint64_t N, uint64_t K, uint64_t Kp, uint32_t L, const int *exps,
+                                  uint32_t tiles, uint32_t tiles_used, uint64_t nks, uint32_t *frag, double *scales,
+                                  hipStream_t stream);
+hipError_t launch_adjoint_i8(const uint8_t *rows, uint64_t stride, uint64_t V, uint64_t N, const uint8_t *flip,
+                             const uint32_t *frag, const double *scales, uint32_t tiles, uint64_t K, uint64_t Kp,
+                             uint32_t L, double *ws, float *dW, hipStream_t stream);
 hipError_t launch_flip_bytes(const uint8_t *flip, uint64_t V, uint64_t Rpad, uint8_t *out, hipStream_t stream);
 constexpr int kI8KStep = 64;           // rows per k-step of the int8 kernel
 // scores[p][k] += sum_s sum_l part[s][p][l*K + k]; counts[p][k] += sum_s cpart[s][p][class[k]]

@@ -174,7 +174,11 @@ gnm_status gnm_dense_forward_device(const uint8_t *d_rows, uint64_t bytes_per_va
 
 /* Adjoint dL/dW = G^T dL/dS, recomputing G from the .bed (no stored
  * dosages).  grad_scores [num_people][num_scores] f32, grad_weights
- * [num_variants][num_scores] f32 (overwritten).  New (no reference counterpart). */
+ * [num_variants][num_scores] f32 (overwritten).  precision: BF16X2/BF16X3
+ * (AUTO = BF16X3) split dL/dS into BF16 limbs per block of 8192 people;
+ * I8X3/I8X4 into int8 limbs per block of 65536 people (rows 4-byte aligned,
+ * bytes_per_variant a multiple of 4).  Up to 64 scores per call.  New (no
+ * reference counterpart). */
 gnm_status gnm_dense_adjoint_device(const uint8_t *d_rows, uint64_t bytes_per_variant,
                                     uint64_t num_variants, uint64_t num_people,
                                     const uint8_t *d_flip, const float *d_grad_scores,

@@ -93,22 +93,36 @@ def test_dense_forward_matches_fp64(orc, precision):
             assert err.max() / np.abs(S64).max() <= 1e-6
 
 
-@pytest.mark.parametrize("precision,K", [("bf16x3", 16), ("bf16x3", 40), ("bf16x2", 16)])
-def test_adjoint_matches_fp64(orc, precision, K):
-    N, V = 9000, 4100                  # two person scale blocks; last workgroup partial
+@pytest.mark.parametrize("precision,K,N,V", [("bf16x3", 16, 9000, 4100), ("bf16x3", 40, 9000, 4100),
+                                              ("bf16x2", 16, 9000, 4100), ("i8x4", 16, 9000, 4100),
+                                              ("i8x4", 40, 9000, 4100), ("i8x3", 16, 9000, 4100),
+                                              ("i8x4", 64, 9000, 300), ("i8x4", 16, 70_000, 300)])
+def test_adjoint_matches_fp64(orc, precision, K, N, V):
+    """N = 9000: two BF16 person blocks, one int8 block, a last super-step
+    that runs past the row end, partial last workgroup; N = 70,000: two int8
+    person blocks (FP64 flush across blocks); K = 64: 16 int8 tiles."""
     _flush(f"adjoint {precision} N={N} V={V} K={K}")
     dos, rows, flip, _ = _data(N, V, 1, 12)
     rng = np.random.default_rng(13)
     R = rng.normal(0, 1, (N, K)).astype(np.float32)
     R[:, 3] *= 1e-6                    # a small column: scales are per column
     dW64 = orc.adjoint_fp64(rows, N, flip, R.astype(np.float64))
+    if precision.startswith("i8"):
+        # the int8 adjoint reads 4-byte aligned rows: pad each row with zero bytes
+        rows = np.pad(rows, ((0, 0), (0, -rows.shape[1] % 4)))
+        assert ag._adjoint_precision(precision, _dev(rows)) == precision
     dW = ag.dense_adjoint(_dev(R), _dev(rows), _dev(flip), V, precision)
     torch.cuda.synchronize()
     dW = dW.cpu().numpy().astype(np.float64)
     G = _effect_dosage(dos, flip).T                   # [N, V]
-    bound = _limb_bound(G, R.astype(np.float64), precision)
+    bound = _limb_bound(G, R.astype(np.float64), precision, I8_BLOCK if precision.startswith("i8") else BLOCK)
     err = np.abs(dW - dW64)
     assert np.all(err <= bound + 2.0 ** -23 * np.abs(dW64) + 1e-30), (err - bound).max()
+
+
+def test_int8_adjoint_falls_back_on_unaligned_rows():
+    bed = torch.zeros(10, 2250, dtype=torch.uint8, device="cuda")   # 2250-byte rows
+    assert ag._adjoint_precision("i8x4", bed) == "bf16x3"
 
 
 def test_adjoint_rejects_too_many_scores():
@@ -128,17 +142,19 @@ def _ridge_problem(N, V, K, seed):
     return dos, rows, flip, W, G, y
 
 
-def test_autograd_matches_finite_differences_and_fp64_gradient(orc):
+@pytest.mark.parametrize("precision", ["bf16x3", "i8x4"])
+def test_autograd_matches_finite_differences_and_fp64_gradient(orc, precision):
     """Ridge-penalised MSE L(W) = sum_k mean_p (S[p,k] - y[p])^2 + lam ||W||^2
-    with S from the split-BF16 forward; gradient through the adjoint."""
+    with S from the split forward of `precision`; gradient through the
+    adjoint of the same arithmetic (500-byte rows: the int8 adjoint runs)."""
     N, V, K, lam = 2000, 3000, 4, 1e-3
-    _flush(f"autograd FD N={N} V={V} K={K}")
+    _flush(f"autograd FD {precision} N={N} V={V} K={K}")
     dos, rows, flip, W0, G, y = _ridge_problem(N, V, K, 21)
     d_rows, d_flip = _dev(rows), _dev(flip)
     y_t = torch.from_numpy(y).cuda()
 
     W = torch.from_numpy(W0).cuda().requires_grad_(True)
-    S = ag.polygenic_score(W, d_rows, d_flip, N, "bf16x3")
+    S = ag.polygenic_score(W, d_rows, d_flip, N, precision)
     loss = ((S - y_t[:, None]) ** 2).mean(0).sum() + lam * (W.double() ** 2).sum()
     loss.backward()
     g_auto = W.grad.detach().cpu().numpy().astype(np.float64)
