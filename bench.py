@@ -162,6 +162,33 @@ def _avg_rule_fraction(S, C, S64, C64, counts, tol=1e-6):
     return float(ok.mean())
 
 
+def measure_host_rows(d_bed, csr, N, V, K, stride, args):
+    """One pass with the .bed rows in pageable host memory, fed through
+    gnm_score_rows: parallel host copy into pinned double-buffered staging,
+    host-to-device copies on their own stream overlapped with scoring.  The
+    PCIe-inclusive rate; best of two passes."""
+    from gnomon_amd.pipeline import ScoringContext
+
+    t0 = time.time()
+    host = d_bed.view(V, stride).cpu().numpy()
+    log(f"host rows: {V * stride / 1e9:.1f} GB copied to host memory ({time.time() - t0:.1f}s)")
+    times = []
+    with ScoringContext(csr, N, precision=args.precision, force=True) as hc:
+        hc.score_rows(host[:min(V, 65536)], 0)    # allocates the staging slots
+        hc.read_accumulators()
+        for _ in range(2):
+            t1 = time.perf_counter()
+            hc.score_rows(host, 0)
+            hc.read_accumulators()                 # waits for the last block
+            times.append(time.perf_counter() - t1)
+    del host
+    t = min(times)
+    log(f"host rows: {t * 1e3:.1f} ms per pass, {V * stride / t / 1e9:.1f} GB/s of rows")
+    return {"ms_per_pass": round(t * 1e3, 2), "value": N * V * K / t, "unit": "sample*variant*score/s",
+            "row_bytes_per_s": V * stride / t,
+            "path": "rows in pageable host memory through gnm_score_rows (pinned 2-slot staging, copy stream)"}
+
+
 def run_precision_report(w, csr, d_bed, stride, args):
     """Per-score error of each precision at full size: against the CPU FP64
     oracle on the first 128 people (all variants), and against the plain FP32
@@ -434,6 +461,8 @@ def main():
     ap.add_argument("--check", action="store_true", help="check a slice against the FP64 oracle")
     ap.add_argument("--no-unpack", action="store_true", help="skip the standalone unpack-stage measurement")
     ap.add_argument("--no-report", action="store_true", help="skip the per-score precision report (precision_64)")
+    ap.add_argument("--host-rows", action="store_true",
+                    help="also time a pass with the rows in host memory (PCIe-inclusive, not the headline value)")
     ap.add_argument("--launch", default="", metavar="TP,BS",
                     help="int8 launch shape (tiles per pass, batch stages) instead of the default")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -546,6 +575,10 @@ def main():
         precision_report = run_precision_report(w, csr, d_bed, stride, args)
     unpack = None if args.no_unpack or w.name != "plain_fp32_biobank" else measure_unpack(d_bed, N, V, bpv, stride, max(3, min(args.steps, 10)), w.name)
 
+    host_rows = None
+    if args.host_rows:
+        host_rows = measure_host_rows(d_bed, csr, N, V, K, stride, args)
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as orc
@@ -573,7 +606,7 @@ def main():
         "config": {"workload": w.name, "people": N, "variants": V, "scores": K,
                    "bed_bytes": V * bpv, "precision": args.precision},
         "roofline": roofline, "cpu_baseline": cpu, "compare": compare, "check": check,
-        "unpack_stage": unpack, "precision_report": precision_report,
+        "unpack_stage": unpack, "precision_report": precision_report, "host_rows": host_rows,
     }
     print(json.dumps(out), flush=True)
 

@@ -112,7 +112,9 @@ struct gnm_ctx {
     uint64_t cbpv = 0;             // compact bytes per variant
     uint8_t *d_stage[2] = {nullptr, nullptr};
     uint8_t *h_stage[2] = {nullptr, nullptr};
-    hipEvent_t stage_free[2] = {nullptr, nullptr};
+    hipEvent_t stage_free[2] = {nullptr, nullptr};    // compute finished with the slot
+    hipEvent_t stage_copied[2] = {nullptr, nullptr};  // the slot's rows are on the device
+    hipStream_t copy_stream = nullptr;                // host-to-device copies of the staging slots
     uint64_t stage_rows = 0;       // capacity of each staging slot in rows
     uint64_t stage_pitch = 0;      // staging row pitch (bytes_per_variant rounded up to 64)
     hipStream_t stream = nullptr;
@@ -546,11 +548,14 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
 void destroy_context(gnm_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int i = 0; i < 2; ++i) {
         if (c->stage_free[i]) (void)hipEventDestroy(c->stage_free[i]);
+        if (c->stage_copied[i]) (void)hipEventDestroy(c->stage_copied[i]);
         if (c->h_stage[i]) (void)hipHostFree(c->h_stage[i]);
     }
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     for (hipEvent_t e : {c->ev_main0, c->ev_main1, c->ev_all0, c->ev_all1})
         if (e) (void)hipEventDestroy(e);
     for (auto it = c->allocations.rbegin(); it != c->allocations.rend(); ++it) (void)hipFree(it->ptr);
@@ -725,15 +730,19 @@ gnm_status gnm_score_rows(gnm_ctx *ctx, const uint8_t *rows, uint64_t row_stride
         const uint64_t bpv = c->bpv;
         if (!c->h_stage[0]) {
             // two pinned slots of ~256 MiB (at least one row) -- the reference's
-            // PIPELINE_SLOTS = 2 double buffer (cuda_backend.rs:42, 1137-1147)
+            // PIPELINE_SLOTS = 2 double buffer (cuda_backend.rs:42, 1137-1147).
+            // Copies run on their own stream (default priority), so the copy of
+            // one slot overlaps the scoring of the other.
             c->stage_pitch = (bpv + 63) / 64 * 64;  // 128-bit aligned rows for the staging loads
             c->stage_rows = std::max<uint64_t>(1, (256ull << 20) / c->stage_pitch);
+            GNM_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
             for (int i = 0; i < 2; ++i) {
                 GNM_HIP(hipHostMalloc((void **)&c->h_stage[i], c->stage_rows * c->stage_pitch, hipHostMallocDefault));
                 std::memset(c->h_stage[i], 0, c->stage_rows * c->stage_pitch);
                 gnm_status s2 = c->alloc(&c->d_stage[i], c->stage_rows * c->stage_pitch);
                 if (s2 != GNM_OK) return s2;
                 GNM_HIP(hipEventCreateWithFlags(&c->stage_free[i], hipEventDisableTiming));
+                GNM_HIP(hipEventCreateWithFlags(&c->stage_copied[i], hipEventDisableTiming));
                 GNM_HIP(hipEventRecord(c->stage_free[i], c->stream));
             }
         }
@@ -741,15 +750,24 @@ gnm_status gnm_score_rows(gnm_ctx *ctx, const uint8_t *rows, uint64_t row_stride
         int slot = 0;
         while (done < num_rows) {
             const uint64_t n = std::min(c->stage_rows, num_rows - done);
+            // the slot's previous rows are scored (and so copied): refill it
             GNM_HIP(hipEventSynchronize(c->stage_free[slot]));
             const uint8_t *src = rows + done * row_stride;
             const uint64_t pitch = c->stage_pitch;
-            if (row_stride == pitch) {
-                std::memcpy(c->h_stage[slot], src, n * pitch);
-            } else {
-                for (uint64_t r = 0; r < n; ++r) std::memcpy(c->h_stage[slot] + r * pitch, src + r * row_stride, bpv);
-            }
-            GNM_HIP(hipMemcpyAsync(c->d_stage[slot], c->h_stage[slot], n * pitch, hipMemcpyHostToDevice, c->stream));
+            uint8_t *dst = c->h_stage[slot];
+            // host copy into pinned memory, split over threads for large blocks
+            const uint64_t rows_per_task = std::max<uint64_t>(1, (1ull << 20) / pitch);
+            parallel_for((n + rows_per_task - 1) / rows_per_task, [&](uint64_t a, uint64_t b) {
+                const uint64_t r0 = a * rows_per_task, r1 = std::min(n, b * rows_per_task);
+                if (row_stride == pitch) {
+                    std::memcpy(dst + r0 * pitch, src + r0 * pitch, (r1 - r0) * pitch);
+                } else {
+                    for (uint64_t r = r0; r < r1; ++r) std::memcpy(dst + r * pitch, src + r * row_stride, bpv);
+                }
+            });
+            GNM_HIP(hipMemcpyAsync(c->d_stage[slot], dst, n * pitch, hipMemcpyHostToDevice, c->copy_stream));
+            GNM_HIP(hipEventRecord(c->stage_copied[slot], c->copy_stream));
+            GNM_HIP(hipStreamWaitEvent(c->stream, c->stage_copied[slot], 0));
             st = score_device_rows(c, c->d_stage[slot], pitch, first_variant + done, n);
             if (st != GNM_OK) return st;
             GNM_HIP(hipEventRecord(c->stage_free[slot], c->stream));
