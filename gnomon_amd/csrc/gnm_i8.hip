@@ -11,7 +11,9 @@
 // represents 23 bits, L = 4 31 bits of the block maximum.
 //
 // Data path, per workgroup of WAVES waves (64 people per wave, 16*WAVES
-// packed bytes of every row) and per stage of 64 program rows (one k-step):
+// packed bytes of every row: 8 waves for 3-tile groups, so a workgroup reads
+// whole 128-byte lines of each row, else 4) and per stage of 64 program rows
+// (one k-step):
 //   * each thread loads 4 rows x 4 packed bytes (one dword per row;
 //     consecutive threads read consecutive dwords of a row), transposes the
 //     4x4 bytes in registers and writes 4 dwords into a column-major LDS tile
@@ -108,7 +110,9 @@ __device__ __forceinline__ void lds_dma_b128(__amdgpu_buffer_rsrc_t rsrc, void *
 }  // namespace
 
 template <int TMAX, int WAVES, int D, bool VEC, bool IDENT>
-__global__ __launch_bounds__(64 * WAVES) void k_score_i8(DevicePlan plan, const uint8_t *__restrict__ rows,
+// The 3-tile, 2-stage instance is asked to fit four waves per SIMD (128
+// registers: a few values move to accumulation registers, none to memory).
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX == 3 && D == 2 ? 4 : 1))) void k_score_i8(DevicePlan plan, const uint8_t *__restrict__ rows,
                                                          uint64_t stride, uint64_t bpv, uint64_t v0, uint64_t p0,
                                                          uint64_t p1, uint64_t split_base, uint64_t rps,
                                                          uint32_t splits, uint32_t person_tiles,
@@ -378,45 +382,40 @@ __global__ __launch_bounds__(64 * WAVES) void k_score_i8(DevicePlan plan, const 
         u32x4_t bv[TMAX];
 #pragma unroll
         for (int j = 0; j < TMAX; ++j) bv[j] = btile[par][buf][j * 64 + lane];
-        uint32_t sel[4][4];   // [person t][dword q]
+        // person tile by person tile: only one tile's selectors and operands
+        // are live at a time
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int tt = 0; tt < 4; ++tt) {
+            uint32_t sel[4];
 #pragma unroll
-            for (int tt = 0; tt < 4; ++tt) sel[tt][q] = ((X[q] >> (2 * tt)) & 0x03030303u) | F[q];
-        auto mfma_tile = [&](int j, const i32x4_t (&op)[4]) __attribute__((always_inline)) {
-            const i32x4_t B = i32x4_t{(int)bv[j][0], (int)bv[j][1], (int)bv[j][2], (int)bv[j][3]};
+            for (int q = 0; q < 4; ++q) sel[q] = ((X[q] >> (2 * tt)) & 0x03030303u) | F[q];
+            i32x4_t A, M;
+            if constexpr (NW != 0) {
 #pragma unroll
-            for (int tt = 0; tt < 4; ++tt)
-                acc[tt][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(op[tt], B, acc[tt][j], 0, 0, 0);
-        };
-        i32x4_t A[4], M[4];
-        if constexpr (NW != 0) {
-#pragma unroll
-            for (int tt = 0; tt < 4; ++tt)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) A[tt][q] = (int)dose_i8(sel[tt][q]);
-        }
-        if constexpr (NW >= 0) {
-            if constexpr (NW < TMAX) {
-#pragma unroll
-                for (int tt = 0; tt < 4; ++tt)
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) M[tt][q] = (int)miss_i8(sel[tt][q]);
+                for (int q = 0; q < 4; ++q) A[q] = (int)dose_i8(sel[q]);
             }
+            auto mfma = [&](int j, const i32x4_t &op) __attribute__((always_inline)) {
+                const i32x4_t B = i32x4_t{(int)bv[j][0], (int)bv[j][1], (int)bv[j][2], (int)bv[j][3]};
+                acc[tt][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(op, B, acc[tt][j], 0, 0, 0);
+            };
+            if constexpr (NW >= 0) {
+                if constexpr (NW < TMAX) {
 #pragma unroll
-            for (int j = 0; j < TMAX; ++j) mfma_tile(j, j < NW ? A : M);
-        } else {
+                    for (int q = 0; q < 4; ++q) M[q] = (int)miss_i8(sel[q]);
+                }
 #pragma unroll
-            for (int j = 0; j < TMAX; ++j)
-                if ((uint32_t)j < nw) mfma_tile(j, A);
-            if (nw < ntile) {
-#pragma unroll
-                for (int tt = 0; tt < 4; ++tt)
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) M[tt][q] = (int)miss_i8(sel[tt][q]);
+                for (int j = 0; j < TMAX; ++j) mfma(j, j < NW ? A : M);
+            } else {
 #pragma unroll
                 for (int j = 0; j < TMAX; ++j)
-                    if ((uint32_t)j >= nw && (uint32_t)j < ntile) mfma_tile(j, M);
+                    if ((uint32_t)j < nw) mfma(j, A);
+                if (nw < ntile) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) M[q] = (int)miss_i8(sel[q]);
+#pragma unroll
+                    for (int j = 0; j < TMAX; ++j)
+                        if ((uint32_t)j >= nw && (uint32_t)j < ntile) mfma(j, M);
+                }
             }
         }
     };
@@ -538,32 +537,37 @@ hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t
     // default takes the cheapest of 3, 4, 6 and 8 by the cost model above
     const uint32_t tmax = tiles_per_pass ? tiles_per_pass : i8_default_tiles_per_pass(ntiles);
     if (tmax != 3 && tmax != 4 && tmax != 6 && tmax != 8) return hipErrorInvalidValue;
-    const uint32_t depth = batch_stages ? batch_stages : 4u;
+    // two stages per batch by default: the smaller register ring lets the
+    // 3-tile instance keep four waves per SIMD
+    const uint32_t depth = batch_stages ? batch_stages : 2u;
     if (depth != 2 && depth != 4) return hipErrorInvalidValue;
     const uint32_t ngroups = (ntiles + tmax - 1) / tmax;
-    constexpr int WV = 4;
-    const uint32_t pt = (uint32_t)((num_people + 64 * WV - 1) / (64 * WV));
+    // 3-tile groups run 8 waves (512 people, 128 bytes of each row per
+    // workgroup: whole 128-byte lines); larger groups 4 waves
+    const uint32_t wv = tmax == 3 ? 8u : 4u;
+    const uint32_t pt = (uint32_t)((num_people + 64 * wv - 1) / (64 * wv));
     const uint64_t wgs = (uint64_t)pt * splits * ngroups;   /* splits is a multiple of 8 */
-#define GNM_LAUNCH_I8(TM, DD, VC, ID)                                                                        \
+#define GNM_LAUNCH_I8(TM, WV, DD, VC, ID)                                                                    \
     hipLaunchKernelGGL((k_score_i8<TM, WV, DD, VC, ID>), dim3((unsigned)wgs), dim3(64 * WV), 0, stream, plan,   \
                        rows, stride, bpv, v0, p0, p1, split_base, rows_per_split, splits, pt, num_people,       \
                        tile_begin, tile_end, ngroups, part, cpart)
-#define GNM_LAUNCH_I8_VI(TM, DD)                                                                             \
+#define GNM_LAUNCH_I8_VI(TM, WV, DD)                                                                         \
     do {                                                                                                     \
         if (vec) {                                                                                           \
-            if (identity) GNM_LAUNCH_I8(TM, DD, true, true); else GNM_LAUNCH_I8(TM, DD, true, false);         \
+            if (identity) GNM_LAUNCH_I8(TM, WV, DD, true, true); else GNM_LAUNCH_I8(TM, WV, DD, true, false); \
         } else {                                                                                             \
-            if (identity) GNM_LAUNCH_I8(TM, DD, false, true); else GNM_LAUNCH_I8(TM, DD, false, false);       \
+            if (identity) GNM_LAUNCH_I8(TM, WV, DD, false, true);                                            \
+            else GNM_LAUNCH_I8(TM, WV, DD, false, false);                                                    \
         }                                                                                                    \
     } while (0)
-#define GNM_LAUNCH_I8_D(TM)                                                                                  \
+#define GNM_LAUNCH_I8_D(TM, WV)                                                                              \
     do {                                                                                                     \
-        if (depth == 2) GNM_LAUNCH_I8_VI(TM, 2); else GNM_LAUNCH_I8_VI(TM, 4);                               \
+        if (depth == 2) GNM_LAUNCH_I8_VI(TM, WV, 2); else GNM_LAUNCH_I8_VI(TM, WV, 4);                       \
     } while (0)
-    if (tmax == 3) GNM_LAUNCH_I8_D(3);
-    else if (tmax == 4) GNM_LAUNCH_I8_D(4);
-    else if (tmax == 6) GNM_LAUNCH_I8_D(6);
-    else GNM_LAUNCH_I8_D(8);
+    if (tmax == 3) GNM_LAUNCH_I8_D(3, 8);
+    else if (tmax == 4) GNM_LAUNCH_I8_D(4, 4);
+    else if (tmax == 6) GNM_LAUNCH_I8_D(6, 4);
+    else GNM_LAUNCH_I8_D(8, 4);
 #undef GNM_LAUNCH_I8_D
 #undef GNM_LAUNCH_I8_VI
 #undef GNM_LAUNCH_I8
