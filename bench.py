@@ -164,29 +164,43 @@ def _avg_rule_fraction(S, C, S64, C64, counts, tol=1e-6):
 
 def measure_host_rows(d_bed, csr, N, V, K, stride, args):
     """One pass with the .bed rows in pageable host memory, fed through
-    gnm_score_rows: parallel host copy into pinned double-buffered staging,
+    gnm_score_rows: parallel host fill of pinned double-buffered staging,
     host-to-device copies on their own stream overlapped with scoring.  The
-    PCIe-inclusive rate; best of two passes."""
+    PCIe-inclusive rate; best of two passes, with the default sparse ingest
+    (rows in which at most half of the 64-bit words hold a non-zero call sent
+    as a bitmap plus those words) and with every row sent packed."""
     from gnomon_amd.pipeline import ScoringContext
 
     t0 = time.time()
     host = d_bed.view(V, stride).cpu().numpy()
     log(f"host rows: {V * stride / 1e9:.1f} GB copied to host memory ({time.time() - t0:.1f}s)")
-    times = []
-    with ScoringContext(csr, N, precision=args.precision, force=True) as hc:
-        hc.score_rows(host[:min(V, 65536)], 0)    # allocates the staging slots
-        hc.read_accumulators()
-        for _ in range(2):
-            t1 = time.perf_counter()
-            hc.score_rows(host, 0)
-            hc.read_accumulators()                 # waits for the last block
-            times.append(time.perf_counter() - t1)
+    res = {}
+    for label, fraction in (("sparse_ingest", 0.5), ("packed_only", 0.0)):
+        times = []
+        with ScoringContext(csr, N, precision=args.precision, force=True) as hc:
+            hc.set_sparse_ingest(fraction)
+            hc.score_rows(host[:min(V, 65536)], 0)    # allocates the staging slots
+            hc.read_accumulators()
+            before = hc.ingest_stats()
+            for _ in range(2):
+                t1 = time.perf_counter()
+                hc.score_rows(host, 0)
+                hc.read_accumulators()                 # waits for the last block
+                times.append(time.perf_counter() - t1)
+            after = hc.ingest_stats()
+        t = min(times)
+        st = {k: (after[k] - before[k]) // 2 for k in after}
+        log(f"host rows [{label}]: {t * 1e3:.1f} ms per pass, {V * stride / t / 1e9:.1f} GB/s of rows, "
+            f"{st['bytes_to_device'] / 1e9:.2f} GB to the device, {st['sparse_rows']} rows sparse")
+        res[label] = {"ms_per_pass": round(t * 1e3, 2), "value": N * V * K / t, "row_bytes_per_s": V * stride / t,
+                      "bytes_to_device": st["bytes_to_device"], "dense_rows": st["dense_rows"],
+                      "sparse_rows": st["sparse_rows"], "stored_words": st["stored_words"]}
     del host
-    t = min(times)
-    log(f"host rows: {t * 1e3:.1f} ms per pass, {V * stride / t / 1e9:.1f} GB/s of rows")
-    return {"ms_per_pass": round(t * 1e3, 2), "value": N * V * K / t, "unit": "sample*variant*score/s",
-            "row_bytes_per_s": V * stride / t,
-            "path": "rows in pageable host memory through gnm_score_rows (pinned 2-slot staging, copy stream)"}
+    out = dict(res["sparse_ingest"])
+    out["unit"] = "sample*variant*score/s"
+    out["packed_only"] = res["packed_only"]
+    out["path"] = "rows in pageable host memory through gnm_score_rows (pinned 2-slot staging, copy stream)"
+    return out
 
 
 def run_precision_report(w, csr, d_bed, stride, args):

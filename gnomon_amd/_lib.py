@@ -35,7 +35,8 @@ EXPORTS = (
     "gnm_create_forced", "gnm_score_rows", "gnm_score_rows_device", "gnm_read_accumulators",
     "gnm_seed_accumulators", "gnm_finish", "gnm_destroy", "gnm_run", "gnm_unpack_device",
     "gnm_dense_forward_device", "gnm_dense_adjoint_device", "gnm_synth_rows_device",
-    "gnm_last_timing", "gnm_set_launch_config",
+    "gnm_last_timing", "gnm_set_launch_config", "gnm_set_sparse_ingest", "gnm_ingest_stats",
+    "gnm_sparse_encode_row",
 )
 
 
@@ -116,6 +117,12 @@ def lib():
     L.gnm_set_launch_config.argtypes = [vp, u32, u32]
     L.gnm_last_timing.restype = st
     L.gnm_last_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+    L.gnm_set_sparse_ingest.restype = st
+    L.gnm_set_sparse_ingest.argtypes = [vp, ctypes.c_double]
+    L.gnm_ingest_stats.restype = st
+    L.gnm_ingest_stats.argtypes = [vp] + [ctypes.POINTER(u64)] * 4
+    L.gnm_sparse_encode_row.restype = ctypes.c_int64
+    L.gnm_sparse_encode_row.argtypes = [vp, u64, vp, u64, vp, vp, vp]
     _lib = L
     return L
 

@@ -123,6 +123,17 @@ struct gnm_ctx {
     uint64_t completed = 0;
     uint32_t tiles_per_pass = 0;   // int8 kernel launch shape (0 = default), gnm_set_launch_config
     uint32_t batch_stages = 0;
+    // sparse ingest of host rows (gnm_set_sparse_ingest)
+    double sparse_fraction = 0.5;         // a row with at most this fraction of non-zero words goes sparse
+    bool sparse_ready = false;            // upload buffers allocated
+    uint32_t sparse_nch = 0;              // 512-word chunks per staged row (pitch / 4096, rounded up)
+    std::vector<uint64_t> keep2;          // [ceil(Nfam/32)] 2-bit masks of the scored people (subsets only)
+    uint8_t *d_pack[2] = {nullptr, nullptr};      // uploaded dense rows + entries
+    uint32_t *h_map[2] = {nullptr, nullptr}, *d_map[2] = {nullptr, nullptr};     // [stage_rows]
+    uint32_t *h_svrow[2] = {nullptr, nullptr}, *d_svrow[2] = {nullptr, nullptr}; // [stage_rows] sparse rows
+    uint64_t *h_bmoff[2] = {nullptr, nullptr}, *d_bmoff[2] = {nullptr, nullptr}; // [stage_rows] their records
+    uint32_t *h_cs[2] = {nullptr, nullptr}, *d_cs[2] = {nullptr, nullptr};       // [stage_rows][nch+1]
+    uint64_t st_dense = 0, st_sparse = 0, st_words = 0, st_bytes = 0;            // gnm_ingest_stats
 
     template <class T>
     gnm_status alloc(T **out, size_t count) {
@@ -220,9 +231,9 @@ inline uint16_t bf16_of_int(int32_t q) {
 }
 
 template <class F>
-void parallel_for(uint64_t n, F &&fn) {
+void parallel_for(uint64_t n, F &&fn, uint64_t min_items = 64) {
     unsigned T = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
-    if (n < 64 || T == 1) {
+    if (n < min_items || T == 1) {
         fn(0, n);
         return;
     }
@@ -258,6 +269,13 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
     c->row_offsets.assign(p->sparse_row_offsets, p->sparse_row_offsets + V + 1);
     c->cols.assign(p->sparse_score_columns, p->sparse_score_columns + nnz);
     c->corr.assign(p->sparse_missing_corrections, p->sparse_missing_corrections + nnz);
+    if (!c->identity) {   // the sparse ingest sends the calls of scored people only
+        c->keep2.assign((c->Nfam + 31) / 32, 0);
+        for (uint64_t i = 0; i < c->N; ++i) {
+            const uint32_t f = p->output_idx_to_fam_idx[i];
+            c->keep2[f / 32] |= 3ull << (2 * (f % 32));
+        }
+    }
 
     // ---- membership classes (identical CSR column supports share a count)
     std::vector<uint64_t> h1(K, 0), h2(K, 0), cnt(K, 0);
@@ -554,6 +572,10 @@ void destroy_context(gnm_ctx *c) {
         if (c->stage_free[i]) (void)hipEventDestroy(c->stage_free[i]);
         if (c->stage_copied[i]) (void)hipEventDestroy(c->stage_copied[i]);
         if (c->h_stage[i]) (void)hipHostFree(c->h_stage[i]);
+        if (c->h_map[i]) (void)hipHostFree(c->h_map[i]);
+        if (c->h_svrow[i]) (void)hipHostFree(c->h_svrow[i]);
+        if (c->h_bmoff[i]) (void)hipHostFree(c->h_bmoff[i]);
+        if (c->h_cs[i]) (void)hipHostFree(c->h_cs[i]);
     }
     if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     for (hipEvent_t e : {c->ev_main0, c->ev_main1, c->ev_all0, c->ev_all1})
@@ -634,6 +656,27 @@ gnm_status score_device_rows(gnm_ctx *c, const uint8_t *d_rows, uint64_t stride,
     GNM_HIP(hipEventRecord(c->ev_all1, c->stream));
     c->timing_valid = true;
     c->completed = v0 + n;
+    return GNM_OK;
+}
+
+// Upload buffers of the sparse ingest, allocated on the first
+// gnm_score_rows call that uses it.
+gnm_status sparse_init(gnm_ctx *c) {
+    c->sparse_nch = (uint32_t)((c->stage_pitch + kSparseChunkWords * 8 - 1) / (kSparseChunkWords * 8));
+    const uint64_t ncs = c->stage_rows * (c->sparse_nch + 1);
+    gnm_status st;
+    for (int i = 0; i < 2; ++i) {
+        if ((st = c->alloc(&c->d_pack[i], c->stage_rows * c->stage_pitch)) != GNM_OK) return st;
+        if ((st = c->alloc(&c->d_map[i], c->stage_rows)) != GNM_OK) return st;
+        if ((st = c->alloc(&c->d_svrow[i], c->stage_rows)) != GNM_OK) return st;
+        if ((st = c->alloc(&c->d_bmoff[i], c->stage_rows)) != GNM_OK) return st;
+        if ((st = c->alloc(&c->d_cs[i], ncs)) != GNM_OK) return st;
+        GNM_HIP(hipHostMalloc((void **)&c->h_map[i], c->stage_rows * sizeof(uint32_t), hipHostMallocDefault));
+        GNM_HIP(hipHostMalloc((void **)&c->h_svrow[i], c->stage_rows * sizeof(uint32_t), hipHostMallocDefault));
+        GNM_HIP(hipHostMalloc((void **)&c->h_bmoff[i], c->stage_rows * sizeof(uint64_t), hipHostMallocDefault));
+        GNM_HIP(hipHostMalloc((void **)&c->h_cs[i], ncs * sizeof(uint32_t), hipHostMallocDefault));
+    }
+    c->sparse_ready = true;
     return GNM_OK;
 }
 
@@ -734,7 +777,7 @@ gnm_status gnm_score_rows(gnm_ctx *ctx, const uint8_t *rows, uint64_t row_stride
             // Copies run on their own stream (default priority), so the copy of
             // one slot overlaps the scoring of the other.
             c->stage_pitch = (bpv + 63) / 64 * 64;  // 128-bit aligned rows for the staging loads
-            c->stage_rows = std::max<uint64_t>(1, (256ull << 20) / c->stage_pitch);
+            c->stage_rows = std::min<uint64_t>(kSparseMaxSlotRows, std::max<uint64_t>(1, (256ull << 20) / c->stage_pitch));
             GNM_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
             for (int i = 0; i < 2; ++i) {
                 GNM_HIP(hipHostMalloc((void **)&c->h_stage[i], c->stage_rows * c->stage_pitch, hipHostMallocDefault));
@@ -746,6 +789,26 @@ gnm_status gnm_score_rows(gnm_ctx *ctx, const uint8_t *rows, uint64_t row_stride
                 GNM_HIP(hipEventRecord(c->stage_free[i], c->stream));
             }
         }
+        const bool use_sparse = c->sparse_fraction > 0.0;
+        if (use_sparse && !c->sparse_ready) {
+            st = sparse_init(c);
+            if (st != GNM_OK) return st;
+        }
+        const uint32_t nch = c->sparse_nch;
+        const uint64_t nbm = (uint64_t)nch * (kSparseChunkWords / 64);   // bitmap words per sparse row
+        // a sparse row (bitmap + stored words) must take less than a packed row
+        const uint64_t row_words = (c->Nfam + 31) / 32;
+        const uint64_t cap = c->stage_pitch > nbm * 8 ? (c->stage_pitch - nbm * 8) / 8 : 0;
+        const uint64_t max_words = std::min<uint64_t>((uint64_t)(c->sparse_fraction * (double)row_words), cap);
+        const bool sparse_on = use_sparse && max_words > 0;
+        const uint64_t *keep2 = c->identity ? nullptr : c->keep2.data();
+        struct TaskOut {
+            std::vector<uint32_t> svrow;   // row of each sparse row of the task
+            std::vector<uint64_t> bmoff;   // word offset of its record (bitmap, then stored words) in the slot
+            std::vector<uint32_t> cs;      // [nch+1] per sparse row, relative to its first stored word
+            uint64_t lo = 0, hi = 0;       // task bytes [0, lo): dense rows; [hi, end): sparse records
+            uint64_t ndense = 0, nwords = 0;
+        };
         uint64_t done = 0;
         int slot = 0;
         while (done < num_rows) {
@@ -754,22 +817,121 @@ gnm_status gnm_score_rows(gnm_ctx *ctx, const uint8_t *rows, uint64_t row_stride
             GNM_HIP(hipEventSynchronize(c->stage_free[slot]));
             const uint8_t *src = rows + done * row_stride;
             const uint64_t pitch = c->stage_pitch;
+            const uint64_t vblk = first_variant + done;
             uint8_t *dst = c->h_stage[slot];
-            // host copy into pinned memory, split over threads for large blocks
-            const uint64_t rows_per_task = std::max<uint64_t>(1, (1ull << 20) / pitch);
-            parallel_for((n + rows_per_task - 1) / rows_per_task, [&](uint64_t a, uint64_t b) {
-                const uint64_t r0 = a * rows_per_task, r1 = std::min(n, b * rows_per_task);
-                if (row_stride == pitch) {
-                    std::memcpy(dst + r0 * pitch, src + r0 * pitch, (r1 - r0) * pitch);
-                } else {
-                    for (uint64_t r = r0; r < r1; ++r) std::memcpy(dst + r * pitch, src + r * row_stride, bpv);
+            // host work split over threads in tasks of ~8 MiB of rows; task t
+            // owns the slot bytes of its rows
+            const uint64_t rows_per_task = std::max<uint64_t>(1, (8ull << 20) / pitch);
+            const uint64_t ntask = (n + rows_per_task - 1) / rows_per_task;
+            uint64_t nd = n, ne = 0, nsv = 0;
+            std::vector<TaskOut> tout;
+            if (sparse_on) {
+                // one pass: a task fills its bytes with its dense rows from the
+                // bottom and the records of its sparse rows from the top (rows
+                // of variants in no score are not sent at all)
+                uint32_t *map = c->h_map[slot];
+                tout.resize(ntask);
+                parallel_for(ntask, [&](uint64_t a, uint64_t b) {
+                    std::vector<uint64_t> tmp(max_words + 64), tbm(nbm);
+                    std::vector<uint32_t> cst(nch + 1);
+                    for (uint64_t t = a; t < b; ++t) {
+                        TaskOut &o = tout[t];
+                        const uint64_t r0 = t * rows_per_task, r1 = std::min(n, r0 + rows_per_task);
+                        uint8_t *reg = dst + r0 * pitch;
+                        uint64_t lo = 0, hi = (r1 - r0) * pitch;   // each row takes at most pitch bytes
+                        for (uint64_t r = r0; r < r1; ++r) {
+                            const uint64_t v = vblk + r;
+                            const uint8_t *row = src + r * row_stride;
+                            map[r] = kMapZero;
+                            if (c->row_offsets[v] == c->row_offsets[v + 1]) continue;
+                            const int64_t k = sparse_encode_row(row, c->Nfam, keep2, max_words, nch, tmp.data(),
+                                                                tbm.data(), cst.data());
+                            if (k < 0) {
+                                std::memcpy(reg + lo, row, bpv);
+                                std::memset(reg + lo + bpv, 0, pitch - bpv);
+                                map[r] = (uint32_t)(r0 + o.ndense++);   // row slot in the upload
+                                lo += pitch;
+                            } else if (k > 0) {
+                                hi -= (nbm + (uint64_t)k) * 8;
+                                std::memcpy(reg + hi, tbm.data(), nbm * 8);
+                                std::memcpy(reg + hi + nbm * 8, tmp.data(), (uint64_t)k * 8);
+                                map[r] = kMapSparse;
+                                o.svrow.push_back((uint32_t)r);
+                                o.bmoff.push_back((r0 * pitch + hi) / 8);
+                                o.cs.insert(o.cs.end(), cst.begin(), cst.end());
+                                o.nwords += (uint64_t)k;
+                            }
+                        }
+                        o.lo = lo;
+                        o.hi = hi;
+                    }
+                }, 2);
+                nd = 0;
+                for (const TaskOut &o : tout) {
+                    nd += o.ndense;
+                    ne += o.nwords;
+                    nsv += o.svrow.size();
                 }
-            });
-            GNM_HIP(hipMemcpyAsync(c->d_stage[slot], dst, n * pitch, hipMemcpyHostToDevice, c->copy_stream));
-            GNM_HIP(hipEventRecord(c->stage_copied[slot], c->copy_stream));
-            GNM_HIP(hipStreamWaitEvent(c->stream, c->stage_copied[slot], 0));
-            st = score_device_rows(c, c->d_stage[slot], pitch, first_variant + done, n);
-            if (st != GNM_OK) return st;
+            }
+            if (nd < n) {
+                uint64_t bytes = 0, s0 = 0;
+                uint32_t *svrow = c->h_svrow[slot], *cs = c->h_cs[slot];
+                uint64_t *bmoff = c->h_bmoff[slot];
+                for (uint64_t t = 0; t < ntask; ++t) {
+                    const TaskOut &o = tout[t];
+                    const uint64_t base = t * rows_per_task * pitch;
+                    const uint64_t end = (std::min(n, (t + 1) * rows_per_task) - t * rows_per_task) * pitch;
+                    if (o.lo)
+                        GNM_HIP(hipMemcpyAsync(c->d_pack[slot] + base, dst + base, o.lo, hipMemcpyHostToDevice,
+                                               c->copy_stream));
+                    if (o.hi < end)
+                        GNM_HIP(hipMemcpyAsync(c->d_pack[slot] + base + o.hi, dst + base + o.hi, end - o.hi,
+                                               hipMemcpyHostToDevice, c->copy_stream));
+                    bytes += o.lo + (end - o.hi);
+                    std::memcpy(svrow + s0, o.svrow.data(), o.svrow.size() * sizeof(uint32_t));
+                    std::memcpy(bmoff + s0, o.bmoff.data(), o.bmoff.size() * sizeof(uint64_t));
+                    std::memcpy(cs + s0 * (nch + 1), o.cs.data(), o.cs.size() * sizeof(uint32_t));
+                    s0 += o.svrow.size();
+                }
+                GNM_HIP(hipMemcpyAsync(c->d_map[slot], c->h_map[slot], n * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                       c->copy_stream));
+                if (nsv) {
+                    GNM_HIP(hipMemcpyAsync(c->d_svrow[slot], svrow, nsv * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                           c->copy_stream));
+                    GNM_HIP(hipMemcpyAsync(c->d_bmoff[slot], bmoff, nsv * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                           c->copy_stream));
+                    GNM_HIP(hipMemcpyAsync(c->d_cs[slot], cs, nsv * (nch + 1) * sizeof(uint32_t),
+                                           hipMemcpyHostToDevice, c->copy_stream));
+                }
+                GNM_HIP(hipEventRecord(c->stage_copied[slot], c->copy_stream));
+                GNM_HIP(hipStreamWaitEvent(c->stream, c->stage_copied[slot], 0));
+                GNM_HIP(launch_expand_rows(c->d_pack[slot], c->d_map[slot], n, pitch, c->d_stage[slot], c->stream));
+                GNM_HIP(launch_sparse_rows((const uint64_t *)c->d_pack[slot], c->d_bmoff[slot], c->d_svrow[slot],
+                                           c->d_cs[slot], nsv, nch, pitch, c->d_stage[slot], c->stream));
+                st = score_device_rows(c, c->d_stage[slot], pitch, vblk, n);
+                if (st != GNM_OK) return st;
+                c->st_bytes += bytes + n * sizeof(uint32_t) + nsv * (nch + 4) * sizeof(uint32_t);   // + metadata
+            } else {
+                // every row dense: the rows as they are (a sparse pass already
+                // wrote them in order)
+                if (!sparse_on) parallel_for(ntask, [&](uint64_t a, uint64_t b) {
+                    const uint64_t r0 = a * rows_per_task, r1 = std::min(n, b * rows_per_task);
+                    if (row_stride == pitch) {
+                        std::memcpy(dst + r0 * pitch, src + r0 * pitch, (r1 - r0) * pitch);
+                    } else {
+                        for (uint64_t r = r0; r < r1; ++r) std::memcpy(dst + r * pitch, src + r * row_stride, bpv);
+                    }
+                }, 2);
+                GNM_HIP(hipMemcpyAsync(c->d_stage[slot], dst, n * pitch, hipMemcpyHostToDevice, c->copy_stream));
+                GNM_HIP(hipEventRecord(c->stage_copied[slot], c->copy_stream));
+                GNM_HIP(hipStreamWaitEvent(c->stream, c->stage_copied[slot], 0));
+                st = score_device_rows(c, c->d_stage[slot], pitch, vblk, n);
+                if (st != GNM_OK) return st;
+                c->st_bytes += n * pitch;
+            }
+            c->st_dense += nd;
+            c->st_sparse += nsv;
+            c->st_words += ne;
             GNM_HIP(hipEventRecord(c->stage_free[slot], c->stream));
             done += n;
             slot ^= 1;
@@ -888,6 +1050,32 @@ gnm_status gnm_set_launch_config(gnm_ctx *ctx, uint32_t tiles_per_pass, uint32_t
     ctx->tiles_per_pass = tiles_per_pass;
     ctx->batch_stages = batch_stages;
     return GNM_OK;
+}
+
+gnm_status gnm_set_sparse_ingest(gnm_ctx *ctx, double max_word_fraction) {
+    if (!ctx) return fail(GNM_ERR_INVALID_ARG, "ctx is NULL");
+    if (!(max_word_fraction >= 0.0 && max_word_fraction <= 0.5))
+        return fail(GNM_ERR_INVALID_ARG, "max_word_fraction must lie in [0, 1/2]");
+    ctx->sparse_fraction = max_word_fraction;
+    return GNM_OK;
+}
+
+gnm_status gnm_ingest_stats(gnm_ctx *ctx, uint64_t *dense_rows, uint64_t *sparse_rows, uint64_t *stored_words,
+                            uint64_t *bytes_to_device) {
+    if (!ctx || !dense_rows || !sparse_rows || !stored_words || !bytes_to_device)
+        return fail(GNM_ERR_INVALID_ARG, "NULL argument");
+    *dense_rows = ctx->st_dense;
+    *sparse_rows = ctx->st_sparse;
+    *stored_words = ctx->st_words;
+    *bytes_to_device = ctx->st_bytes;
+    return GNM_OK;
+}
+
+int64_t gnm_sparse_encode_row(const uint8_t *row, uint64_t num_fam, const uint64_t *keep2, uint64_t max_words,
+                              uint64_t *words, uint64_t *bitmap, uint32_t *chunk_start) {
+    if ((!row && num_fam) || !words || !bitmap || !chunk_start) return -2;
+    const uint32_t nch = (uint32_t)((num_fam + 32ull * kSparseChunkWords - 1) / (32ull * kSparseChunkWords));
+    return sparse_encode_row(row, num_fam, keep2, max_words, nch, words, bitmap, chunk_start);
 }
 
 gnm_status gnm_synth_rows_device(uint64_t seed, uint32_t p_lo24, uint32_t p_hi24, uint32_t miss16,

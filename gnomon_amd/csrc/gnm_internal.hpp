@@ -98,6 +98,29 @@ hipError_t launch_adjoint_i8(const uint8_t *rows, uint64_t stride, uint64_t V, u
                              uint32_t L, double *ws, float *dW, hipStream_t stream);
 hipError_t launch_flip_bytes(const uint8_t *flip, uint64_t V, uint64_t Rpad, uint8_t *out, hipStream_t stream);
 constexpr int kI8KStep = 64;           // rows per k-step of the int8 kernel
+
+// Sparse ingest (gnm_sparse.hip).  A sparse row goes as a bitmap of its
+// non-zero 64-bit words (32 calls each) plus those words, in chunks of
+// kSparseChunkWords words (4096 row bytes, 16384 people).
+constexpr uint32_t kSparseChunkWords = 512;
+constexpr uint64_t kSparseMaxSlotRows = 1u << 20;   // staged rows per slot
+constexpr uint32_t kMapZero = 0xFFFFFFFFu;           // row map: an all-00 row
+constexpr uint32_t kMapSparse = 0xFFFFFFFEu;         // row map: rebuilt from its words
+// Non-zero words of one packed row (num_fam people; keep2: 2-bit mask per
+// person, NULL = all) in row order; bitmap [nch][8] of the non-zero words;
+// chunk_start[c] (c <= nch) = index of chunk c's first stored word.  out must
+// hold max_words + 64 words.  -1 when there are more than max_words.
+int64_t sparse_encode_row(const uint8_t *row, uint64_t num_fam, const uint64_t *keep2, uint64_t max_words,
+                          uint32_t nch, uint64_t *out, uint64_t *bitmap, uint32_t *chunk_start);
+// out row r (pitch bytes, a multiple of 64) = pack row map[r], zeros for
+// kMapZero; rows marked kMapSparse are left alone
+hipError_t launch_expand_rows(const uint8_t *pack, const uint32_t *map, uint64_t n, uint64_t pitch, uint8_t *out,
+                              hipStream_t stream);
+// rebuilds rows sv_row[s] of out from their records at word bmoff[s] of pack
+// (bitmap [nch][8], then the stored words); cs: [nsv][nch+1] index of each
+// chunk's first stored word within the record's words
+hipError_t launch_sparse_rows(const uint64_t *pack, const uint64_t *bmoff, const uint32_t *sv_row, const uint32_t *cs,
+                              uint64_t nsv, uint32_t nch, uint64_t pitch, uint8_t *out, hipStream_t stream);
 // scores[p][k] += sum_s sum_l part[s][p][l*K + k]; counts[p][k] += sum_s cpart[s][p][class[k]]
 hipError_t launch_reduce(const double *part, const uint32_t *cpart, uint32_t splits,
                          uint64_t num_people, uint64_t num_scores, uint32_t limbs,

@@ -82,6 +82,19 @@ class ScoringContext:
         """Launch shape of the split-int8 kernel (results do not depend on it)."""
         check(lib().gnm_set_launch_config(self._ctx, tiles_per_pass, batch_stages))
 
+    def set_sparse_ingest(self, max_word_fraction: float) -> None:
+        """Rows of score_rows in which at most this fraction of the 64-bit
+        words hold a non-zero call cross PCIe as a bitmap plus those words
+        (0 = every row packed; default 1/2).  Results do not change."""
+        check(lib().gnm_set_sparse_ingest(self._ctx, float(max_word_fraction)))
+
+    def ingest_stats(self) -> dict:
+        """Totals of score_rows since creation: rows sent packed, rows sent
+        sparse, their stored words, bytes copied to the device."""
+        v = [ctypes.c_uint64() for _ in range(4)]
+        check(lib().gnm_ingest_stats(self._ctx, *(ctypes.byref(x) for x in v)))
+        return dict(zip(("dense_rows", "sparse_rows", "stored_words", "bytes_to_device"), (int(x.value) for x in v)))
+
     def last_timing(self):
         a, b = ctypes.c_float(), ctypes.c_float()
         check(lib().gnm_last_timing(self._ctx, ctypes.byref(a), ctypes.byref(b)))

@@ -70,6 +70,11 @@ WORKLOADS = {
     "batching_200": Workload("batching_200", 100_000, 1_200_000, 200, 0.01, 0.5, 0.02, "normal",
                              0.01, files=200, file_min=1_000),
     "adjoint_16": Workload("adjoint_16", 50_000, 500_000, 16, 0.01, 0.5, 0.02, "laplace", 1e-3),
+    # sequencing-style rows for the sparse ingest of host rows: rare variants
+    # (A2 frequency ~U(1e-4, 0.02), 0.1% missing: half the rows sparse) and
+    # very rare ones (~U(1e-5, 0.002), 0.05% missing: every row sparse)
+    "wgs_rare": Workload("wgs_rare", 100_000, 1_000_000, 8, 1e-4, 0.02, 0.001, "normal", 0.01),
+    "wgs_ultrarare": Workload("wgs_ultrarare", 100_000, 1_000_000, 8, 1e-5, 0.002, 0.0005, "normal", 0.01),
 }
 
 

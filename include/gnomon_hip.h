@@ -124,6 +124,34 @@ gnm_status gnm_score_rows(gnm_ctx *ctx, const uint8_t *rows, uint64_t row_stride
 gnm_status gnm_score_rows_device(gnm_ctx *ctx, const uint8_t *d_rows, uint64_t row_stride,
                                  uint64_t first_variant, uint64_t num_rows);
 
+/* Sparse ingest of gnm_score_rows, the device counterpart of the CPU
+ * engine's density dispatch (assess_variant_density_for_dispatch and
+ * run_variant_major_path, score/batch.rs:583-760; path choice
+ * score/decide.rs:100-189).  A host row in which at most max_word_fraction
+ * of the 64-bit words (32 calls each) hold a non-zero call of a scored
+ * person crosses PCIe as a bitmap of those words plus the words, and the
+ * device rebuilds the packed row before scoring it; rows of variants in no
+ * score are not sent.  Results are bit-identical to sending every row
+ * packed.  max_word_fraction in [0, 1/2]; 0 sends every row packed.
+ * Default 1/2.  Device-resident rows (gnm_score_rows_device) are read as
+ * they are. */
+gnm_status gnm_set_sparse_ingest(gnm_ctx *ctx, double max_word_fraction);
+/* Totals since gnm_create of the rows gnm_score_rows sent packed, the rows
+ * sent sparse, their stored words, and the bytes copied to the device. */
+gnm_status gnm_ingest_stats(gnm_ctx *ctx, uint64_t *dense_rows, uint64_t *sparse_rows, uint64_t *stored_words,
+                            uint64_t *bytes_to_device);
+/* The host encoder of the sparse ingest (host only, no device use): the
+ * non-zero 64-bit words of one packed row of num_fam people (keep2: 2-bit
+ * mask per person, 32 people per word, NULL = all; padding calls of the
+ * last byte are dropped) in row order, with a bitmap of them, 8 words of
+ * bitmap per chunk of 512 row words, and chunk_start[c] = index of the first
+ * stored word of chunk c, for c = 0 .. ceil(num_fam / 16384).  Returns the
+ * number of stored words, -1 when there are more than max_words (the row
+ * then goes packed), -2 for NULL buffers.  words must hold max_words + 64
+ * values. */
+int64_t gnm_sparse_encode_row(const uint8_t *row, uint64_t num_fam, const uint64_t *keep2, uint64_t max_words,
+                              uint64_t *words, uint64_t *bitmap, uint32_t *chunk_start);
+
 /* Checkpoint contract (score/checkpoint.rs:293-339, cuda_backend.rs:1969-2011):
  * exact f64 sums (baseline included, as the reference stores them) and u32
  * counts for variants [0, completed_variants). */
