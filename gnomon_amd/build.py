@@ -6,7 +6,7 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("gnm_kernels.hip", "gnm_dense.hip", "gnm_i8.hip", "gnm_sparse.hip", "gnm_api.hip")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("gnm_kernels.hip", "gnm_dense.hip", "gnm_i8.hip", "gnm_sparse.hip", "gnm_plan.hip", "gnm_api.hip")]
 OUTPUT = os.path.join(HERE, "libgnomon_hip.so")
 
 # -packed-fp32-ops off: hipcc -O3 may otherwise form v_pk_{add,mul,fma}_f32 from

@@ -191,17 +191,6 @@ gnm_status validate(const gnm_problem *p) {
     return GNM_OK;
 }
 
-// Signed base-256 digits of an integer with |Q| <= 2^(8L) - 2^(8(L-1)):
-// low digits in [-128, 127], top digit in [-255, 255]; all exact in BF16.
-inline void limb_digits(int64_t Q, uint32_t L, int32_t *out) {
-    for (uint32_t j = L - 1; j > 0; --j) {
-        int64_t d = ((Q + 128) & 255) - 128;
-        out[j] = (int32_t)d;
-        Q = (Q - d) / 256;
-    }
-    out[0] = (int32_t)Q;
-}
-
 inline bool is_int8(int precision) {
     return precision == GNM_PRECISION_I8X3 || precision == GNM_PRECISION_I8X4;
 }
@@ -222,13 +211,6 @@ inline double limb_qmax(uint32_t L, bool int8) {
     return int8 ? 127.0 * std::ldexp(1.0, 8 * (int)(L - 1)) : std::ldexp(1.0, 8 * (int)L) - std::ldexp(1.0, 8 * (int)(L - 1));
 }
 
-inline uint16_t bf16_of_int(int32_t q) {
-    // |q| <= 256: exactly representable; float -> bf16 by truncation is exact.
-    float f = (float)q;
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-    return (uint16_t)(u >> 16);
-}
 
 template <class F>
 void parallel_for(uint64_t n, F &&fn, uint64_t min_items = 64) {
@@ -246,6 +228,34 @@ void parallel_for(uint64_t n, F &&fn, uint64_t min_items = 64) {
     }
     for (auto &x : th) x.join();
 }
+
+// Device buffers needed only while gnm_create builds the plan.
+struct BuildScratch {
+    std::vector<void *> ptrs;
+    ~BuildScratch() {
+        for (void *q : ptrs) (void)hipFree(q);
+    }
+    template <class T>
+    gnm_status get(T **out, size_t count) {
+        void *q = nullptr;
+        const hipError_t e = hipMalloc(&q, std::max<size_t>(count * sizeof(T), 256));
+        if (e != hipSuccess)
+            return fail(e == hipErrorOutOfMemory ? GNM_ERR_OUT_OF_MEMORY : GNM_ERR_HIP,
+                        std::string("hipMalloc (plan build): ") + hipGetErrorString(e));
+        ptrs.push_back(q);
+        *out = (T *)q;
+        return GNM_OK;
+    }
+    template <class T>
+    gnm_status put(T **out, const T *host, size_t count) {
+        gnm_status st = get(out, count);
+        if (st == GNM_OK && count) {
+            const hipError_t e = hipMemcpy(*out, host, count * sizeof(T), hipMemcpyHostToDevice);
+            if (e != hipSuccess) return fail(GNM_ERR_HIP, std::string("hipMemcpy (plan build): ") + hipGetErrorString(e));
+        }
+        return st;
+    }
+};
 
 gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
     c->N = p->num_people;
@@ -346,22 +356,6 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
         prog_cls[r] = m;  // first row of the variant is its count row
     }
 
-    // ---- dense effect-allele weight rows (double), row-major [R][K]
-    std::vector<double> wrow(Rpad * K, 0.0);
-    parallel_for(V, [&](uint64_t a, uint64_t b) {
-        for (uint64_t v = a; v < b; ++v) {
-            const uint64_t r0 = c->var_prog_off[v];
-            const bool has_d = var_rows[v] & 1, has_e = var_rows[v] & 2;
-            const uint64_t rd = r0, re = has_d ? r0 + 1 : r0;
-            for (uint64_t e = c->row_offsets[v]; e < c->row_offsets[v + 1]; ++e) {
-                const double we = p->sparse_weights[e], co = p->sparse_missing_corrections[e];
-                const uint32_t k = c->cols[e];
-                if (has_d) wrow[rd * K + k] += we + 0.5 * co;
-                if (has_e) wrow[re * K + k] += 0.5 * co;
-            }
-        }
-    });
-
     // ---- device
     GNM_HIP(hipSetDevice(c->device));
     GNM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -392,150 +386,68 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
     const bool mfma_weights = c->precision != GNM_PRECISION_FP32;
     c->fp32_count_in_kernel = c->nclass <= (uint32_t)kFp32MaxClasses;
     const bool mfma_counts = mfma_weights || !c->fp32_count_in_kernel;
-
-    if (!mfma_weights) {
-        const uint64_t k32 = (K + kFp32ScoreChunk - 1) / kFp32ScoreChunk * kFp32ScoreChunk;
-        // FP32 weights (rounded once from the effect-allele row value), held as f64
-        std::vector<double> w64(Rpad * k32, 0.0);
-        for (uint64_t r = 0; r < R; ++r)
-            for (uint64_t k = 0; k < K; ++k) w64[r * k32 + k] = (double)(float)wrow[r * K + k];
-        double *d_w64;
-        if ((st = c->upload(&d_w64, w64)) != GNM_OK) return st;
-        pl.w64 = d_w64;
-        pl.k32 = k32;
-    }
     const uint32_t L = c->limbs;
     pl.ncol = mfma_weights ? (uint32_t)(L * K) : 0;
     pl.wtiles = (pl.ncol + 15) / 16;
     pl.ctiles = mfma_counts ? (c->nclass + 15) / 16 : 0;
     const uint32_t tiles = pl.wtiles + pl.ctiles;
-    if (tiles) {
-        const uint64_t kstep = int8 ? (uint64_t)kI8KStep : (uint64_t)kKStep;
-        const uint64_t nks = Rpad / kstep;
-        const uint64_t fr = (uint64_t)flush_rows(int8);   // rows per scale block
-        const uint64_t nblk = (Rpad + fr - 1) / fr;
-        std::vector<double> scales(mfma_weights ? nblk * pl.ncol : 1, 0.0);
-        // per (scale block, score): u = 2^e with max|w|/u <= 2^(8L) - 2^(8(L-1))
-        std::vector<int> uexp(mfma_weights ? nblk * K : 1, 0);
-        if (mfma_weights) {
-            const double qmax = limb_qmax(L, int8);
-            parallel_for(nblk, [&](uint64_t a, uint64_t b) {
-                for (uint64_t blk = a; blk < b; ++blk) {
-                    const uint64_t r0 = blk * fr, r1 = std::min<uint64_t>(R, r0 + fr);
-                    for (uint64_t k = 0; k < K; ++k) {
-                        double m = 0.0;
-                        for (uint64_t r = r0; r < r1; ++r) m = std::max(m, std::fabs(wrow[r * K + k]));
-                        int e = 0;
-                        if (m > 0.0) {
-                            e = (int)std::ceil(std::log2(m / qmax));
-                            while (std::ldexp(qmax, e) < m) ++e;
-                            while (std::ldexp(qmax, e - 1) >= m) --e;
-                        }
-                        uexp[blk * K + k] = e;
-                        for (uint32_t j = 0; j < L; ++j)
-                            scales[blk * pl.ncol + j * K + k] = std::ldexp(1.0, e + 8 * (int)(L - 1 - j));
-                    }
-                }
-            });
+
+    // ---- the weight plan, built on the device from the CSR (gnm_plan.hip):
+    // f64 effect-allele rows, then scales and B fragments or FP32 weights
+    {
+        BuildScratch sc;
+        uint64_t *d_off, *d_prog_off;
+        uint32_t *d_cols, *d_rowcls = nullptr;
+        float *d_w, *d_co;
+        uint8_t *d_var_rows, *d_count_row = nullptr;
+        double *d_wrow;
+        if ((st = sc.put(&d_off, c->row_offsets.data(), V + 1)) != GNM_OK) return st;
+        if ((st = sc.put(&d_cols, c->cols.data(), nnz)) != GNM_OK) return st;
+        if ((st = sc.put(&d_w, p->sparse_weights, nnz)) != GNM_OK) return st;
+        if ((st = sc.put(&d_co, p->sparse_missing_corrections, nnz)) != GNM_OK) return st;
+        if ((st = sc.put(&d_prog_off, c->var_prog_off.data(), V + 1)) != GNM_OK) return st;
+        if ((st = sc.put(&d_var_rows, var_rows.data(), V)) != GNM_OK) return st;
+        if ((st = sc.get(&d_wrow, Rpad * K)) != GNM_OK) return st;
+        GNM_HIP(launch_fill_f64(d_wrow, Rpad * K, 0.0, c->stream));
+        const uint32_t cw = (c->nclass + 31) / 32;
+        if (pl.ctiles) {
+            if ((st = sc.get(&d_rowcls, std::max<uint64_t>(R, 1) * cw)) != GNM_OK) return st;
+            GNM_HIP(launch_fill_u32(d_rowcls, std::max<uint64_t>(R, 1) * cw, 0u, c->stream));
+            if ((st = sc.put(&d_count_row, prog_count_row.data(), Rpad)) != GNM_OK) return st;
         }
-        if (int8) {
-            // [ks][tile][lane][16 i8]; lane (n = lane%16, g = lane/16) byte b
-            // holds row 64ks + 16g + b of column tile*16 + n.  One pass over
-            // (row, score): the digits of a weight are computed once and each
-            // limb placed in its column; count rows set their class columns.
-            std::vector<uint8_t> frag8(nks * tiles * 64 * 16, 0);
-            const uint32_t cw = (c->nclass + 31) / 32;
-            std::vector<uint32_t> rowcls(pl.ctiles ? R * cw : 0, 0);
-            if (pl.ctiles)
-                for (uint64_t v = 0; v < V; ++v) {
-                    if (!var_rows[v]) continue;
-                    const uint64_t r = c->var_prog_off[v];   // count row of the variant
-                    for (uint64_t e = c->row_offsets[v]; e < c->row_offsets[v + 1]; ++e) {
-                        const uint32_t cl = c->score_class[c->cols[e]];
-                        rowcls[r * cw + cl / 32] |= 1u << (cl % 32);
-                    }
-                }
-            parallel_for(nks, [&](uint64_t a, uint64_t b) {
-                std::vector<int32_t> dig(L);
-                for (uint64_t ks = a; ks < b; ++ks) {
-                    const uint64_t blk = ks * kstep / fr;
-                    for (uint32_t rr = 0; rr < kstep; ++rr) {
-                        const uint64_t r = ks * kstep + rr;
-                        if (r >= R) break;
-                        const uint32_t g = rr / 16, byte = rr % 16;
-                        auto put = [&](uint32_t col, int32_t v) {
-                            const uint32_t tile = col / 16, lane = (col % 16) + 16 * g;
-                            frag8[((ks * tiles + tile) * 64 + lane) * 16 + byte] = (uint8_t)(int8_t)v;
-                        };
-                        for (uint64_t k = 0; k < K; ++k) {
-                            const double w = wrow[r * K + k];
-                            if (w == 0.0) continue;
-                            const double Qd = std::nearbyint(std::ldexp(w, -uexp[blk * K + k]));
-                            limb_digits((int64_t)Qd, L, dig.data());
-                            for (uint32_t l = 0; l < L; ++l) put(l * (uint32_t)K + (uint32_t)k, dig[l]);
-                        }
-                        if (pl.ctiles && prog_count_row[r])
-                            for (uint32_t wd = 0; wd < cw; ++wd)
-                                for (uint32_t bits = rowcls[r * cw + wd]; bits; bits &= bits - 1)
-                                    put(pl.wtiles * 16 + wd * 32 + (uint32_t)__builtin_ctz(bits), 1);
-                    }
-                }
-            });
-            uint8_t *d_frag8;
-            if ((st = c->upload(&d_frag8, frag8)) != GNM_OK) return st;
-            pl.bfrag = (const uint32_t *)d_frag8;
-        } else {
-        // B fragments: [ks][tile][lane][8 bf16]; lane (n = lane%16, g = lane/16)
-        // holds rows 32ks + 8g + j (j = 0..7) of column tile*16 + n.
-        std::vector<uint16_t> frag(nks * tiles * 64 * 8, 0);
-        parallel_for(nks, [&](uint64_t a, uint64_t b) {
-            std::vector<int32_t> dig(L);
-            for (uint64_t ks = a; ks < b; ++ks) {
-                const uint64_t blk = ks * kKStep / kFlushRows;
-                for (uint32_t tile = 0; tile < tiles; ++tile) {
-                    uint16_t *dst = frag.data() + ((ks * tiles + tile) * 64) * 8;
-                    for (int lane = 0; lane < 64; ++lane) {
-                        const uint32_t n = lane & 15, g = lane >> 4;
-                        for (int j = 0; j < 8; ++j) {
-                            const uint64_t r = ks * kKStep + g * 8 + j;
-                            uint16_t val = 0;
-                            if (r < R) {
-                                if (tile < pl.wtiles) {
-                                    const uint32_t colw = tile * 16 + n;
-                                    if (colw < pl.ncol) {
-                                        const uint32_t limb = colw / (uint32_t)K;
-                                        const uint64_t k = colw % K;
-                                        const double w = wrow[r * K + k];
-                                        if (w != 0.0) {
-                                            const double Qd = std::nearbyint(std::ldexp(w, -uexp[blk * K + k]));
-                                            limb_digits((int64_t)Qd, L, dig.data());
-                                            val = bf16_of_int(dig[limb]);
-                                        }
-                                    }
-                                } else {
-                                    const uint32_t cl = (tile - pl.wtiles) * 16 + n;
-                                    if (cl < c->nclass && prog_count_row[r]) {
-                                        const uint64_t v = prog_var[r];
-                                        for (uint64_t e = c->row_offsets[v]; e < c->row_offsets[v + 1]; ++e)
-                                            if (c->score_class[c->cols[e]] == cl) { val = 0x3F80; break; }
-                                    }
-                                }
-                            }
-                            dst[lane * 8 + j] = val;
-                        }
-                    }
-                }
+        GNM_HIP(launch_plan_rows(d_off, d_cols, d_w, d_co, d_prog_off, d_var_rows, V, K, c->d_score_class, cw, d_wrow,
+                                 d_rowcls, c->stream));
+        if (!mfma_weights) {
+            const uint64_t k32 = (K + kFp32ScoreChunk - 1) / kFp32ScoreChunk * kFp32ScoreChunk;
+            double *d_w64;
+            if ((st = c->alloc(&d_w64, Rpad * k32)) != GNM_OK) return st;
+            GNM_HIP(launch_fill_f64(d_w64, Rpad * k32, 0.0, c->stream));
+            GNM_HIP(launch_plan_w64(d_wrow, R, K, k32, d_w64, c->stream));
+            pl.w64 = d_w64;
+            pl.k32 = k32;
+        }
+        if (tiles) {
+            const uint64_t kstep = int8 ? (uint64_t)kI8KStep : (uint64_t)kKStep;
+            const uint64_t nks = Rpad / kstep;
+            const uint64_t fr = (uint64_t)flush_rows(int8);   // rows per scale block
+            const uint64_t nblk = (Rpad + fr - 1) / fr;
+            int *d_uexp = nullptr;
+            if (mfma_weights) {
+                double *d_scales;
+                if ((st = sc.get(&d_uexp, nblk * K)) != GNM_OK) return st;
+                if ((st = c->alloc(&d_scales, nblk * pl.ncol)) != GNM_OK) return st;
+                GNM_HIP(launch_plan_exps(d_wrow, R, K, fr, limb_qmax(L, int8), L, pl.ncol, nblk, d_uexp, d_scales,
+                                         c->stream));
+                pl.scales = d_scales;
             }
-        });
-        uint16_t *d_frag;
-        if ((st = c->upload(&d_frag, frag)) != GNM_OK) return st;
-        pl.bfrag = (const uint32_t *)d_frag;
+            // 16 bytes per (k-step, tile, lane): 16 int8 or 8 BF16 values
+            uint4 *d_frag;
+            if ((st = c->alloc(&d_frag, nks * tiles * 64)) != GNM_OK) return st;
+            GNM_HIP(launch_plan_frag(int8, d_wrow, R, K, d_uexp, fr, L, pl.ncol, pl.wtiles, tiles, nks, d_count_row,
+                                     d_rowcls, cw, c->nclass, d_frag, c->stream));
+            pl.bfrag = (const uint32_t *)d_frag;
         }
-        if (mfma_weights) {
-            double *d_scales;
-            if ((st = c->upload(&d_scales, scales)) != GNM_OK) return st;
-            pl.scales = d_scales;
-        }
+        GNM_HIP(hipStreamSynchronize(c->stream));
     }
 
     // ---- accumulators and partials

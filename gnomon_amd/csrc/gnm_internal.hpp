@@ -25,6 +25,22 @@ constexpr int kFp32ScoreChunk = 8;     // scores per pass of the FP32 kernel
 constexpr int kFp32MaxClasses = 4;     // membership classes counted in-kernel by the FP32 kernel
 constexpr int kMfmaMaxTiles = 8;       // 16-column tiles per pass of the MFMA kernel
 
+// Signed base-256 digits of an integer with |Q| <= 2^(8L) - 2^(8(L-1)):
+// low digits in [-128, 127], top digit in [-255, 255]; all exact in BF16.
+__host__ __device__ inline void limb_digits(int64_t Q, uint32_t L, int32_t *out) {
+    for (uint32_t j = L - 1; j > 0; --j) {
+        const int64_t d = ((Q + 128) & 255) - 128;
+        out[j] = (int32_t)d;
+        Q = (Q - d) / 256;
+    }
+    out[0] = (int32_t)Q;
+}
+
+// |q| <= 256 is exact in BF16: the top half of its f32 pattern
+__host__ __device__ inline uint16_t bf16_of_int(int32_t q) {
+    return (uint16_t)(__builtin_bit_cast(uint32_t, (float)q) >> 16);
+}
+
 // Device-side view of the weight plan built by gnm_create.
 struct DevicePlan {
     uint64_t num_prog_rows;    // R: reduction rows (one per variant, two for mixed-flip variants)
@@ -98,6 +114,26 @@ hipError_t launch_adjoint_i8(const uint8_t *rows, uint64_t stride, uint64_t V, u
                              uint32_t L, double *ws, float *dW, hipStream_t stream);
 hipError_t launch_flip_bytes(const uint8_t *flip, uint64_t V, uint64_t Rpad, uint8_t *out, hipStream_t stream);
 constexpr int kI8KStep = 64;           // rows per k-step of the int8 kernel
+
+// Device build of the weight plan (gnm_plan.hip).  wrow: f64 [R][K]
+// (zeroed by the caller); rowcls: u32 [R][cw] class bits of count rows
+// (zeroed; NULL when no count tiles).
+hipError_t launch_plan_rows(const uint64_t *off, const uint32_t *cols, const float *w, const float *co,
+                            const uint64_t *prog_off, const uint8_t *var_rows, uint64_t V, uint64_t K,
+                            const uint32_t *score_class, uint32_t cw, double *wrow, uint32_t *rowcls,
+                            hipStream_t stream);
+// per (scale block of fr rows, score): exponent e with max|w| <= qmax * 2^e
+// and the limb scales [block][L*K]
+hipError_t launch_plan_exps(const double *wrow, uint64_t R, uint64_t K, uint64_t fr, double qmax, uint32_t L,
+                            uint64_t ncol, uint64_t nblk, int *uexp, double *scales, hipStream_t stream);
+// B fragments of nks k-steps (int8: 64 rows, [ks][tile][lane][16 i8];
+// BF16: 32 rows, [ks][tile][lane][8 bf16])
+hipError_t launch_plan_frag(bool int8, const double *wrow, uint64_t R, uint64_t K, const int *uexp, uint64_t fr,
+                            uint32_t L, uint32_t ncol, uint32_t wtiles, uint32_t tiles, uint64_t nks,
+                            const uint8_t *count_row, const uint32_t *rowcls, uint32_t cw, uint32_t nclass,
+                            void *frag, hipStream_t stream);
+// FP32-path weights w64[r][k] (row pitch k32) = W rounded to f32
+hipError_t launch_plan_w64(const double *wrow, uint64_t R, uint64_t K, uint64_t k32, double *w64, hipStream_t stream);
 
 // Sparse ingest (gnm_sparse.hip).  A sparse row goes as a bitmap of its
 // non-zero 64-bit words (32 calls each) plus those words, in chunks of
