@@ -537,9 +537,10 @@ hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t
     // default takes the cheapest of 3, 4, 6 and 8 by the cost model above
     const uint32_t tmax = tiles_per_pass ? tiles_per_pass : i8_default_tiles_per_pass(ntiles);
     if (tmax != 3 && tmax != 4 && tmax != 6 && tmax != 8) return hipErrorInvalidValue;
-    // two stages per batch by default: the smaller register ring lets the
-    // 3-tile instance keep four waves per SIMD
-    const uint32_t depth = batch_stages ? batch_stages : 2u;
+    // stages per batch by default: two for 3-tile groups (the smaller
+    // register ring keeps four waves per SIMD), 4 and 8; four for 6-tile
+    // groups (more loads in flight; 6% faster at 100k x 1M x 64)
+    const uint32_t depth = batch_stages ? batch_stages : (tmax == 6 ? 4u : 2u);
     if (depth != 2 && depth != 4) return hipErrorInvalidValue;
     const uint32_t ngroups = (ntiles + tmax - 1) / tmax;
     // 3-tile groups run 8 waves (512 people, 128 bytes of each row per
