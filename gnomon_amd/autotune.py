@@ -22,7 +22,7 @@ from __future__ import annotations
 import json
 import os
 
-CANDIDATES = [(3, 4), (3, 2), (4, 4), (4, 2), (6, 4), (6, 2), (8, 4), (8, 2)]
+CANDIDATES = [(3, 4), (3, 2), (4, 4), (4, 2), (6, 4), (6, 2), (8, 4), (8, 2), (18, 2)]
 
 _memory: dict[str, tuple[int, int]] = {}
 

@@ -109,9 +109,12 @@ __device__ __forceinline__ void lds_dma_b128(__amdgpu_buffer_rsrc_t rsrc, void *
 
 }  // namespace
 
-template <int TMAX, int WAVES, int D, bool VEC, bool IDENT>
+template <int TMAX, int WAVES, int D, bool VEC, bool IDENT, int SETS>
 // The 3-tile, 2-stage instance is asked to fit four waves per SIMD (128
 // registers: a few values move to accumulation registers, none to memory).
+// SETS > 1: the waves form SETS sets over the same people, each set with its
+// own group of TMAX tiles, so one row read serves SETS groups (the first
+// set loads and stores the rows for all).
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX == 3 && D == 2 ? 4 : 1))) void k_score_i8(DevicePlan plan, const uint8_t *__restrict__ rows,
                                                          uint64_t stride, uint64_t bpv, uint64_t v0, uint64_t p0,
                                                          uint64_t p1, uint64_t split_base, uint64_t rps,
@@ -120,17 +123,20 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
                                                          uint32_t tile_end, uint32_t ngroups,
                                                          double *__restrict__ part,
                                                          uint32_t *__restrict__ cpart) {
-    constexpr int kBytes = 16 * WAVES;               // packed bytes per row per workgroup
+    static_assert(WAVES % SETS == 0, "whole sets of waves");
+    constexpr int PW = WAVES / SETS;                 // waves over distinct people
+    constexpr int kBytes = 16 * PW;                  // packed bytes per row per workgroup
     constexpr int kCols = kBytes + 1;                // + the selector column
     constexpr int kTileWords = kCols * kColPitch / 4;
-    constexpr int kBW = (TMAX + WAVES - 1) / WAVES;  // B tiles each wave fetches per stage
+    constexpr int kBT = SETS * TMAX;                 // B tiles per stage (all sets)
+    constexpr int kBW = (kBT + WAVES - 1) / WAVES;   // B tiles each wave fetches per stage
     static_assert(kI8FlushRows % (D * kI8Stage) == 0, "scale blocks must hold whole batches");
     // one LDS tile per stage of a batch: all D stages are stored, then one
     // barrier, then all D are consumed (two barriers per batch of D stages)
     __shared__ uint32_t tile[D][kTileWords];
     // B fragments are double-buffered: the next batch's DMA lands in one
     // half while this batch's MFMAs read the other
-    __shared__ u32x4_t btile[2][D][TMAX * 64];
+    __shared__ u32x4_t btile[2][D][kBT * 64];
 
     // Workgroups are dispatched round-robin over the 8 XCDs (L mod 8).  One
     // launch covers every tile group: the groups of one (split, person tile)
@@ -143,15 +149,20 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
     const uint32_t s = xcd + 8u * (rest / person_tiles);
     const uint32_t ptile = rest % person_tiles;
     if (s >= splits) return;
-    const uint32_t tile_begin = tile_begin0 + group * TMAX;
-    const uint32_t ntile = tile_end - tile_begin < (uint32_t)TMAX ? tile_end - tile_begin : (uint32_t)TMAX;
+    const uint32_t gtile0 = tile_begin0 + group * kBT;   // first tile of the workgroup's sets
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int pwave = wave % PW, set = wave / PW;
+    // the first set loads and stores the rows of every stage
+    const bool loader = SETS == 1 || wave < PW;
+    const uint32_t tile_begin = gtile0 + (uint32_t)set * TMAX;
+    const uint32_t ntile = tile_begin >= tile_end ? 0u
+                           : (tile_end - tile_begin < (uint32_t)TMAX ? tile_end - tile_begin : (uint32_t)TMAX);
     const int li = lane & 15, lg = lane >> 4;
     const uint64_t col0 = (uint64_t)ptile * kBytes;
-    const uint64_t pw0 = (uint64_t)ptile * (64 * WAVES) + (uint64_t)wave * 64;
+    const uint64_t pw0 = (uint64_t)ptile * (64 * PW) + (uint64_t)pwave * 64;
     const uint64_t sb = split_base + (uint64_t)s * rps;
     const uint64_t r_begin = p0 > sb ? p0 : sb;
     const uint64_t se = sb + rps;
@@ -163,7 +174,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
     // load role: rows 4rg..4rg+3, packed bytes 4c4..4c4+3 of the workgroup's span
     // (consecutive threads take consecutive dwords of a row: each load
     // instruction reads 64 contiguous bytes of each of its rows)
-    const int rg = tid / (4 * WAVES), c4 = tid % (4 * WAVES);
+    const int rg = tid / (4 * PW), c4 = tid % (4 * PW);
     const uint64_t ld_col = col0 + 4 * (uint64_t)c4;
     const bool ld_any = ld_col < bpv;
     // Every stage issues the same loads on every path (one dword per row):
@@ -188,16 +199,16 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
     const uint32_t row_voff = (uint32_t)(4 * (uint64_t)rg * stride + ld_col);
 
     const uint64_t last_st = ((r_end - 1) / kI8Stage) * kI8Stage;
-    // B fragments: wave w fetches tiles w, w + WAVES, ... of the group by
-    // LDS DMA (lane l's 16 bytes land at btile[d][64 j + l]); tiles past
-    // ntile are fetched clamped to the last tile and their products are
-    // never flushed
+    // B fragments: wave w fetches tiles w, w + WAVES, ... of the sets' kBT
+    // tiles by LDS DMA (lane l's 16 bytes land at btile[d][64 j + l]); tiles
+    // past tile_end are fetched clamped to the last tile and their products
+    // are never flushed
     uint32_t boff[kBW];     // byte offsets within a stage's fragment block
 #pragma unroll
     for (int m = 0; m < kBW; ++m) {
-        const uint32_t j = (uint32_t)(wave + m * WAVES);
-        const uint32_t jc = j < ntile ? j : ntile - 1;
-        boff[m] = ((tile_begin + jc) * 64 + lane) * 16;
+        const uint32_t j = gtile0 + (uint32_t)(wave + m * WAVES);
+        const uint32_t jc = j < tile_end ? j : tile_end - 1;
+        boff[m] = (jc * 64 + lane) * 16;
     }
     auto fetch_b = [&](uint32_t par, int d, uint64_t st) __attribute__((always_inline)) {
         const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
@@ -205,7 +216,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
 #pragma unroll
         for (int m = 0; m < kBW; ++m) {
             const int j = wave + m * WAVES;
-            if (kBW * WAVES == TMAX || j < TMAX)
+            if (kBW * WAVES == kBT || j < kBT)
                 lds_dma_b128(rb, &btile[par][d][j * 64], boff[m]);
         }
     };
@@ -377,11 +388,11 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
     auto compute = [&](Acc &acc, auto nwc, uint32_t par, int buf) __attribute__((always_inline)) {
         constexpr int NW = decltype(nwc)::value;
         const uint32_t *t = tile[buf];
-        const u32x4_t X = *(const u32x4_t *)&t[tile_dword(wave * 16 + li, 4 * lg)];
+        const u32x4_t X = *(const u32x4_t *)&t[tile_dword(pwave * 16 + li, 4 * lg)];
         const u32x4_t F = *(const u32x4_t *)&t[tile_dword(kBytes, 4 * lg)];
         u32x4_t bv[TMAX];
 #pragma unroll
-        for (int j = 0; j < TMAX; ++j) bv[j] = btile[par][buf][j * 64 + lane];
+        for (int j = 0; j < TMAX; ++j) bv[j] = btile[par][buf][(set * TMAX + j) * 64 + lane];
         // person tile by person tile: only one tile's selectors and operands
         // are live at a time
 #pragma unroll
@@ -434,6 +445,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
     const uint64_t nst = (r_end - st0 + kI8Stage - 1) / kI8Stage;
     auto issue_batch = [&](uint32_t par, uint64_t i) __attribute__((always_inline)) {
         // stages past the split are loaded clamped and never stored
+        if (!loader) return;
         [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
             (issue(par, I, st0 + (i + I) * kI8Stage), ...);
         }(std::make_integer_sequence<int, D>{});
@@ -460,9 +472,11 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
             const uint64_t blk_end = ((blk + 1) * kI8FlushRows - st0) / kI8Stage;   // a multiple of D
             const uint64_t i_end = blk_end < nst ? blk_end : nst;
             do {   // i0 < i_end on entry: the block holds stage i0
-                [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
-                    ((i0 + I < nst ? (store(I, I), 0) : 0), ...);
-                }(std::make_integer_sequence<int, D>{});
+                if (loader) {
+                    [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
+                        ((i0 + I < nst ? (store(I, I), 0) : 0), ...);
+                    }(std::make_integer_sequence<int, D>{});
+                }
                 dma_lds_barrier();   // this batch's B fragments landed and its tiles are stored
                 issue_batch(par ^ 1u, i0 + D);
                 [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
@@ -534,41 +548,49 @@ hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t
     rows_per_split = (rows_per_split + kI8Stage - 1) / kI8Stage * kI8Stage;
     const uint32_t ntiles = tile_end - tile_begin;
     // tiles per workgroup pass: every group computes tmax tile slots, so the
-    // default takes the cheapest of 3, 4, 6 and 8 by the cost model above
-    const uint32_t tmax = tiles_per_pass ? tiles_per_pass : i8_default_tiles_per_pass(ntiles);
-    if (tmax != 3 && tmax != 4 && tmax != 6 && tmax != 8) return hipErrorInvalidValue;
+    // default takes the cheapest of 3, 4, 6 and 8 by the cost model above;
+    // 18 is two sets of 9-tile groups over the same people (one row read
+    // for both)
+    const uint32_t tp = tiles_per_pass ? tiles_per_pass : i8_default_tiles_per_pass(ntiles);
+    if (tp != 3 && tp != 4 && tp != 6 && tp != 8 && tp != 18) return hipErrorInvalidValue;
     // stages per batch by default: two for 3-tile groups (the smaller
-    // register ring keeps four waves per SIMD), 4 and 8; four for 6-tile
-    // groups (more loads in flight; 6% faster at 100k x 1M x 64)
-    const uint32_t depth = batch_stages ? batch_stages : (tmax == 6 ? 4u : 2u);
+    // register ring keeps four waves per SIMD), 4 and 8 and the two-set
+    // shape; four for 6-tile groups (more loads in flight; 6% faster at
+    // 100k x 1M x 64)
+    const uint32_t depth = batch_stages ? batch_stages : (tp == 6 ? 4u : 2u);
     if (depth != 2 && depth != 4) return hipErrorInvalidValue;
-    const uint32_t ngroups = (ntiles + tmax - 1) / tmax;
+    if (tp == 18 && depth != 2) return hipErrorInvalidValue;   // LDS: two-stage batches only
+    const uint32_t ngroups = (ntiles + tp - 1) / tp;
     // 3-tile groups run 8 waves (512 people, 128 bytes of each row per
-    // workgroup: whole 128-byte lines); larger groups 4 waves
-    const uint32_t wv = tmax == 3 ? 8u : 4u;
-    const uint32_t pt = (uint32_t)((num_people + 64 * wv - 1) / (64 * wv));
+    // workgroup: whole 128-byte lines); larger groups 4 waves; the two-set
+    // shape 2 x 4 waves over 256 people
+    const uint32_t wv = (tp == 3 || tp == 18) ? 8u : 4u;
+    const uint32_t people_per_wg = tp == 18 ? 256u : 64u * wv;
+    const uint32_t pt = (uint32_t)((num_people + people_per_wg - 1) / people_per_wg);
     const uint64_t wgs = (uint64_t)pt * splits * ngroups;   /* splits is a multiple of 8 */
-#define GNM_LAUNCH_I8(TM, WV, DD, VC, ID)                                                                    \
-    hipLaunchKernelGGL((k_score_i8<TM, WV, DD, VC, ID>), dim3((unsigned)wgs), dim3(64 * WV), 0, stream, plan,   \
-                       rows, stride, bpv, v0, p0, p1, split_base, rows_per_split, splits, pt, num_people,       \
+#define GNM_LAUNCH_I8(TM, WV, DD, VC, ID, SE)                                                                \
+    hipLaunchKernelGGL((k_score_i8<TM, WV, DD, VC, ID, SE>), dim3((unsigned)wgs), dim3(64 * WV), 0, stream,     \
+                       plan, rows, stride, bpv, v0, p0, p1, split_base, rows_per_split, splits, pt, num_people, \
                        tile_begin, tile_end, ngroups, part, cpart)
-#define GNM_LAUNCH_I8_VI(TM, WV, DD)                                                                         \
+#define GNM_LAUNCH_I8_VI(TM, WV, DD, SE)                                                                     \
     do {                                                                                                     \
         if (vec) {                                                                                           \
-            if (identity) GNM_LAUNCH_I8(TM, WV, DD, true, true); else GNM_LAUNCH_I8(TM, WV, DD, true, false); \
+            if (identity) GNM_LAUNCH_I8(TM, WV, DD, true, true, SE);                                         \
+            else GNM_LAUNCH_I8(TM, WV, DD, true, false, SE);                                                 \
         } else {                                                                                             \
-            if (identity) GNM_LAUNCH_I8(TM, WV, DD, false, true);                                            \
-            else GNM_LAUNCH_I8(TM, WV, DD, false, false);                                                    \
+            if (identity) GNM_LAUNCH_I8(TM, WV, DD, false, true, SE);                                        \
+            else GNM_LAUNCH_I8(TM, WV, DD, false, false, SE);                                                \
         }                                                                                                    \
     } while (0)
 #define GNM_LAUNCH_I8_D(TM, WV)                                                                              \
     do {                                                                                                     \
-        if (depth == 2) GNM_LAUNCH_I8_VI(TM, WV, 2); else GNM_LAUNCH_I8_VI(TM, WV, 4);                       \
+        if (depth == 2) GNM_LAUNCH_I8_VI(TM, WV, 2, 1); else GNM_LAUNCH_I8_VI(TM, WV, 4, 1);                 \
     } while (0)
-    if (tmax == 3) GNM_LAUNCH_I8_D(3, 8);
-    else if (tmax == 4) GNM_LAUNCH_I8_D(4, 4);
-    else if (tmax == 6) GNM_LAUNCH_I8_D(6, 4);
-    else GNM_LAUNCH_I8_D(8, 4);
+    if (tp == 3) GNM_LAUNCH_I8_D(3, 8);
+    else if (tp == 4) GNM_LAUNCH_I8_D(4, 4);
+    else if (tp == 6) GNM_LAUNCH_I8_D(6, 4);
+    else if (tp == 8) GNM_LAUNCH_I8_D(8, 4);
+    else GNM_LAUNCH_I8_VI(9, 8, 2, 2);
 #undef GNM_LAUNCH_I8_D
 #undef GNM_LAUNCH_I8_VI
 #undef GNM_LAUNCH_I8

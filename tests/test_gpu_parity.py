@@ -319,7 +319,7 @@ def test_int8_scale_blocks_and_launch_shapes(orc, precision):
     csr = csr_from_dense(W, rng.random((V, K)) < 0.5, present)
     S64, C64 = orc.score_fp64(rows, csr, N)
     first = None
-    for tp, bs in [(0, 0), (3, 2), (3, 4), (4, 4), (6, 2), (6, 4), (8, 2), (8, 4)]:
+    for tp, bs in [(0, 0), (3, 2), (3, 4), (4, 4), (6, 2), (6, 4), (8, 2), (8, 4), (18, 2)]:
         _flush(f"  tiles_per_pass={tp} batch_stages={bs}")
         with ScoringContext(csr, N, precision=precision, force=True) as ctx:
             ctx.set_launch_config(tp, bs)
@@ -332,3 +332,28 @@ def test_int8_scale_blocks_and_launch_shapes(orc, precision):
         else:
             assert np.array_equal(C, first[1])
             assert np.array_equal(S, first[0]), f"shape {tp}x{bs} differs"
+
+
+def test_int8_two_set_shape_many_scores(orc):
+    """The two-set shape (two 9-tile groups over the same people, one row read
+    for both) with 40 scores: 10 weight tiles and 3 count tiles, so the second
+    set mixes weight and count tiles; 1,300 people (a ragged last person
+    tile), two scale blocks.  Bit-identical to the default shape and within
+    the i8x4 rule of the FP64 oracle."""
+    _flush("int8 two-set shape")
+    rng = np.random.default_rng(37)
+    N, V, K = 1300, 70_000, 40
+    rows = orc.synth_rows(9, N, 0, V, int(0.02 * 2**24), int(0.5 * 2**24), 1311)
+    W = rng.laplace(0, 1e-3, (V, K)).astype(np.float32)
+    W[rng.random((V, K)) < 0.3] = 0.0
+    csr = csr_from_dense(W, rng.random((V, K)) < 0.5, rng.random((V, K)) < 0.85)
+    S64, C64 = orc.score_fp64(rows, csr, N)
+    out = {}
+    for tp in (0, 18):
+        with ScoringContext(csr, N, precision="i8x4", force=True) as ctx:
+            ctx.set_launch_config(tp, 2 if tp else 0)
+            ctx.score_rows(rows, 0)
+            out[tp] = ctx.finish()
+    _check_scores(*out[0], S64, C64, "i8x4")
+    assert np.array_equal(out[18][1], out[0][1])
+    assert np.array_equal(out[18][0], out[0][0])
