@@ -89,6 +89,7 @@ struct gnm_ctx {
     bool identity = true;
     // host copies needed after create
     std::vector<uint64_t> var_prog_off;   // [V+1]
+    std::vector<uint64_t> non_single;     // [V+1] variants before v with other than one program row
     std::vector<uint64_t> row_offsets;    // [V+1] CSR
     std::vector<uint32_t> cols;           // [nnz]
     std::vector<float> corr;              // [nnz]
@@ -310,6 +311,7 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
 
     // ---- effect-allele program rows
     c->var_prog_off.assign(V + 1, 0);
+    c->non_single.assign(V + 1, 0);
     std::vector<uint8_t> var_rows(V, 0);  // bit0: A2 row, bit1: A1 row
     for (uint64_t v = 0; v < V; ++v) {
         bool need_d = false, need_e = false;
@@ -323,6 +325,7 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
         uint8_t m = empty ? 0 : (need_e ? (need_d ? 3 : 2) : 1);
         var_rows[v] = m;
         c->var_prog_off[v + 1] = c->var_prog_off[v] + (m == 3 ? 2 : (m ? 1 : 0));
+        c->non_single[v + 1] = c->non_single[v] + ((m == 1 || m == 2) ? 0 : 1);
     }
     const uint64_t R = c->var_prog_off[V];
     c->R = R;
@@ -545,7 +548,10 @@ gnm_status score_device_rows(gnm_ctx *c, const uint8_t *d_rows, uint64_t stride,
         c->completed = v0 + n;
         return GNM_OK;
     }
-    const bool identity = (p1 - p0) == n;  // no mixed-flip variant in this block
+    // program row p0 + i is variant v0 + i only when every variant of the
+    // block has exactly one row (an empty variant and a mixed-flip one
+    // would leave the row count equal to n but shift the rows)
+    const bool identity = c->non_single[v0 + n] == c->non_single[v0];
     GNM_HIP(hipEventRecord(c->ev_main0, c->stream));
     if (c->precision == GNM_PRECISION_FP32) {
         GNM_HIP(launch_score_fp32(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, c->K, S,

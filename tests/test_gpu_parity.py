@@ -357,3 +357,101 @@ def test_int8_two_set_shape_many_scores(orc):
     _check_scores(*out[0], S64, C64, "i8x4")
     assert np.array_equal(out[18][1], out[0][1])
     assert np.array_equal(out[18][0], out[0][0])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "i8x4"])
+@pytest.mark.parametrize("N,V,K", [(1, 65, 1), (3, 7, 2), (5, 130, 3), (33, 1, 2), (63, 200, 5)])
+def test_tiny_and_ragged_shapes(orc, precision, N, V, K):
+    """Single person, fewer than 4 people (one partial byte), one variant,
+    person counts straddling 32 and 64; every genotype code present."""
+    _flush(f"tiny {precision} N={N} V={V} K={K}")
+    rng = np.random.default_rng(N * 1000 + V + K)
+    dos = rng.choice([0, 1, 2, -1], size=(V, N), p=[0.4, 0.3, 0.2, 0.1])
+    rows = pack_bed(dos)
+    W = rng.normal(0, 0.05, (V, K)).astype(np.float32)
+    csr = csr_from_dense(W, rng.random((V, K)) < 0.5, rng.random((V, K)) < 0.8)
+    S64, C64 = orc.score_fp64(rows, csr, N)
+    S, C = _score(csr, rows, N, precision)
+    _check_scores(S, C, S64, C64, precision)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "i8x4"])
+def test_degenerate_inputs(orc, precision):
+    """Variants in no score, a score with no variants, an all-missing row, an
+    all-00 row, a score whose weights are all zero (its missing counts still
+    count), weights spanning 1e-30 .. 1e+3, an empty row block."""
+    _flush(f"degenerate {precision}")
+    rng = np.random.default_rng(77)
+    N, V, K = 301, 3000, 6
+    dos = rng.choice([0, 1, 2, -1], size=(V, N), p=[0.45, 0.3, 0.2, 0.05])
+    dos[10] = -1
+    dos[11] = 0
+    rows = pack_bed(dos)
+    W = rng.normal(0, 0.01, (V, K)).astype(np.float32)
+    W[:, 2] = 0.0                                        # all-zero score
+    W[100:200, 3] = np.float32(1e-30)                    # tiny weights
+    W[200:210, 4] = np.float32(1e3)                      # large weights
+    present = rng.random((V, K)) < 0.9
+    present[:, 5] = False                                # a score with no variants
+    present[500:600] = False                             # variants in no score
+    csr = csr_from_dense(W, rng.random((V, K)) < 0.5, present)
+    S64, C64 = orc.score_fp64(rows, csr, N)
+    with ScoringContext(csr, N, precision=precision, force=True) as ctx:
+        ctx.score_rows(rows[:0], 0)                      # empty block: no-op
+        ctx.score_rows(rows[:1500], 0)
+        ctx.score_rows(rows[1500:1500], 1500)
+        ctx.score_rows(rows[1500:], 1500)
+        S, C = ctx.finish()
+    assert np.array_equal(C, C64)
+    assert (C[:, 5] == 0).all() and (S[:, 5] == 0).all()
+    assert (C[:, 2] > 0).any()
+    if precision == "fp32":
+        ok = reference_agree(S, S64, 1e-6)
+        assert ok.all()
+    else:
+        # normwise per score, skipping the empty score
+        assert normwise_rel(S[:, :5], S64[:, :5]).max() <= TOL[precision]
+
+
+def test_forty_people_hundred_scores_partial_overlap(orc):
+    """40 people x 100 scores with partial overlap (each score covers a random
+    40-90% of 30,000 variants) and per-cell effect-allele flips: the shape of
+    the reference's forty_genomes_hundred_scores test (score/tests/
+    gpu_tests.rs:322-411), above the device size gate through N*K*V."""
+    _flush("forty x hundred")
+    rng = np.random.default_rng(101)
+    N, V, K = 40, 30_000, 100
+    rows = orc.synth_rows(12, N, 0, V, int(0.05 * 2**24), int(0.5 * 2**24), 1311)
+    W = rng.normal(0, 0.02, (V, K)).astype(np.float32)
+    cover = rng.uniform(0.4, 0.9, K)
+    present = rng.random((V, K)) < cover[None, :]
+    csr = csr_from_dense(W, rng.random((V, K)) < 0.5, present)
+    assert _lib.lib().gnm_size_gate(N, K, V) == 1
+    S64, C64 = orc.score_fp64(rows, csr, N)
+    Sr, Cr, _ = orc.refcpu_run(rows, csr, N, threads=4)
+    for precision in ("fp32", "i8x4"):
+        S, C = _score(csr, rows, N, precision, blocks=2)
+        _check_scores(S, C, S64, C64, precision)
+        assert np.array_equal(C, Cr)
+        assert reference_agree(S, Sr, 2e-6).all()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "i8x4"])
+def test_empty_and_mixed_flip_variants_in_one_block(orc, precision):
+    """A variant in no score (no program row) and a variant whose cells mix
+    effect alleles (two program rows) in the same block: the row count equals
+    the variant count, but row i is not variant i."""
+    _flush(f"empty+mixed {precision}")
+    rng = np.random.default_rng(5)
+    N, V, K = 200, 64, 2
+    dos = rng.choice([0, 1, 2, -1], size=(V, N), p=[0.4, 0.3, 0.2, 0.1])
+    rows = pack_bed(dos)
+    W = rng.normal(0, 0.05, (V, K)).astype(np.float32)
+    flip = np.zeros((V, K), bool)
+    flip[9, 0] = True                     # variant 9: score 0 reads A1, score 1 reads A2
+    present = np.ones((V, K), bool)
+    present[5] = False                    # variant 5 is in no score
+    csr = csr_from_dense(W, flip, present)
+    S64, C64 = orc.score_fp64(rows, csr, N)
+    S, C = _score(csr, rows, N, precision)
+    _check_scores(S, C, S64, C64, precision)
