@@ -455,3 +455,33 @@ def test_empty_and_mixed_flip_variants_in_one_block(orc, precision):
     S64, C64 = orc.score_fp64(rows, csr, N)
     S, C = _score(csr, rows, N, precision)
     _check_scores(S, C, S64, C64, precision)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "i8x4"])
+def test_device_rows_odd_stride_unaligned_base_and_subset(orc, precision):
+    """Device-resident rows at an odd stride (bytes_per_variant itself) and
+    from an odd base address take the byte-load kernel variants; the same
+    rows with a person subset go through the on-device compaction."""
+    _flush(f"device odd stride {precision}")
+    rng = np.random.default_rng(61)
+    Nfam, V, K = 1001, 4000, 5
+    dos = rng.choice([0, 1, 2, -1], size=(V, Nfam), p=[0.4, 0.3, 0.25, 0.05])
+    rows = pack_bed(dos)                              # stride = bpv = 251
+    W = rng.normal(0, 0.02, (V, K)).astype(np.float32)
+    csr = csr_from_dense(W, rng.random((V, K)) < 0.5, rng.random((V, K)) < 0.9)
+    buf = torch.zeros(rows.size + 1, dtype=torch.uint8, device="cuda")
+    buf[1:] = torch.from_numpy(rows.reshape(-1)).cuda()     # rows start at an odd address
+    base = buf.data_ptr() + 1
+    torch.cuda.synchronize()
+    S64, C64 = orc.score_fp64(rows, csr, Nfam)
+    with ScoringContext(csr, Nfam, precision=precision, force=True) as ctx:
+        ctx.score_rows_device(base, rows.shape[1], 0, 1700)
+        ctx.score_rows_device(base + 1700 * rows.shape[1], rows.shape[1], 1700, V - 1700)
+        S, C = ctx.finish()
+    _check_scores(S, C, S64, C64, precision)
+    o2f = rng.permutation(Nfam)[:777].astype(np.uint32)
+    S64s, C64s = orc.score_fp64(rows, csr, len(o2f), o2f)
+    with ScoringContext(csr, len(o2f), Nfam, o2f, precision=precision, force=True) as ctx:
+        ctx.score_rows_device(base, rows.shape[1], 0, V)
+        S, C = ctx.finish()
+    _check_scores(S, C, S64s, C64s, precision)
