@@ -101,3 +101,44 @@ def test_unscored_people_are_cleared(orc):
 
 def test_null_buffers():
     assert lib().gnm_sparse_encode_row(None, 10, None, 4, None, None, None) == -2
+
+
+def test_vector_and_portable_encoders_agree(tmp_path):
+    """The AVX2 encoder (used where the host has AVX2) and the portable one
+    (forced with GNM_SPARSE_SCALAR=1, read once per process) give the same
+    words, bitmaps and chunk starts, with and without a person mask."""
+    import json
+    import os
+    import subprocess
+    import sys
+
+    script = r'''
+import json, sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from tests.test_sparse_encode import _encode, _keep2
+from tests.helpers import pack_bed
+out = []
+rng = np.random.default_rng(11)
+for nfam, d in [(5, 0.5), (130, 0.05), (16417, 0.003), (40000, 0.02), (33000, 0.2)]:
+    dos = np.where(rng.random(nfam) < d, rng.choice([1, 2, -1], nfam), 0)
+    row = pack_bed(dos[None, :])[0]
+    kept = np.sort(rng.permutation(nfam)[: nfam // 2])
+    for keep in (None, _keep2(nfam, kept)):
+        n, w, b, c = _encode(row, nfam, (nfam + 31) // 32, keep)
+        out.append([int(n), [int(x) for x in w], [int(x) for x in b], [int(x) for x in c]])
+        out.append(int(_encode(row, nfam, max(int(n) - 1, 0), keep)[0]))
+print(json.dumps(out))
+'''
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    res = {}
+    for scalar in (False, True):
+        env = dict(os.environ)
+        env.pop("GNM_SPARSE_SCALAR", None)
+        if scalar:
+            env["GNM_SPARSE_SCALAR"] = "1"
+        p = subprocess.run([sys.executable, "-c", script, root], env=env, capture_output=True, text=True,
+                           timeout=120, cwd=root)
+        assert p.returncode == 0, p.stderr
+        res[scalar] = json.loads(p.stdout.strip().splitlines()[-1])
+    assert res[False] == res[True]
