@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <mutex>
 #include <new>
@@ -221,13 +222,24 @@ void parallel_for(uint64_t n, F &&fn, uint64_t min_items = 64) {
         return;
     }
     std::vector<std::thread> th;
+    // an exception in a worker is carried to the caller (and from there to
+    // the entry point's catch-all) instead of terminating the process
+    std::vector<std::exception_ptr> err(T);
     uint64_t chunk = (n + T - 1) / T;
     for (unsigned t = 0; t < T; ++t) {
         uint64_t a = t * chunk, b = std::min(n, a + chunk);
         if (a >= b) break;
-        th.emplace_back([&fn, a, b] { fn(a, b); });
+        th.emplace_back([&fn, &err, t, a, b] {
+            try {
+                fn(a, b);
+            } catch (...) {
+                err[t] = std::current_exception();
+            }
+        });
     }
     for (auto &x : th) x.join();
+    for (auto &e : err)
+        if (e) std::rethrow_exception(e);
 }
 
 // Device buffers needed only while gnm_create builds the plan.
@@ -856,6 +868,8 @@ gnm_status gnm_score_rows(gnm_ctx *ctx, const uint8_t *rows, uint64_t row_stride
         }
         // the caller may reuse `rows` now: every byte has been copied to pinned staging
         return GNM_OK;
+    } catch (const std::bad_alloc &) {
+        return fail(GNM_ERR_OUT_OF_MEMORY, "host allocation failed in gnm_score_rows");
     } catch (...) {
         return fail(GNM_ERR_INTERNAL, "exception in gnm_score_rows");
     }
