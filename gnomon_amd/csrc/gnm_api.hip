@@ -117,6 +117,7 @@ struct gnm_ctx {
     hipEvent_t stage_free[2] = {nullptr, nullptr};    // compute finished with the slot
     hipEvent_t stage_copied[2] = {nullptr, nullptr};  // the slot's rows are on the device
     hipStream_t copy_stream = nullptr;                // host-to-device copies of the staging slots
+    bool staging_ready = false;    // slots, events and copy stream made
     uint64_t stage_rows = 0;       // capacity of each staging slot in rows
     uint64_t stage_pitch = 0;      // staging row pitch (bytes_per_variant rounded up to 64)
     hipStream_t stream = nullptr;
@@ -595,16 +596,20 @@ gnm_status sparse_init(gnm_ctx *c) {
     c->sparse_nch = (uint32_t)((c->stage_pitch + kSparseChunkWords * 8 - 1) / (kSparseChunkWords * 8));
     const uint64_t ncs = c->stage_rows * (c->sparse_nch + 1);
     gnm_status st;
+    // each resource is made once: a call after a failed allocation resumes
     for (int i = 0; i < 2; ++i) {
-        if ((st = c->alloc(&c->d_pack[i], c->stage_rows * c->stage_pitch)) != GNM_OK) return st;
-        if ((st = c->alloc(&c->d_map[i], c->stage_rows)) != GNM_OK) return st;
-        if ((st = c->alloc(&c->d_svrow[i], c->stage_rows)) != GNM_OK) return st;
-        if ((st = c->alloc(&c->d_bmoff[i], c->stage_rows)) != GNM_OK) return st;
-        if ((st = c->alloc(&c->d_cs[i], ncs)) != GNM_OK) return st;
-        GNM_HIP(hipHostMalloc((void **)&c->h_map[i], c->stage_rows * sizeof(uint32_t), hipHostMallocDefault));
-        GNM_HIP(hipHostMalloc((void **)&c->h_svrow[i], c->stage_rows * sizeof(uint32_t), hipHostMallocDefault));
-        GNM_HIP(hipHostMalloc((void **)&c->h_bmoff[i], c->stage_rows * sizeof(uint64_t), hipHostMallocDefault));
-        GNM_HIP(hipHostMalloc((void **)&c->h_cs[i], ncs * sizeof(uint32_t), hipHostMallocDefault));
+        if (!c->d_pack[i] && (st = c->alloc(&c->d_pack[i], c->stage_rows * c->stage_pitch)) != GNM_OK) return st;
+        if (!c->d_map[i] && (st = c->alloc(&c->d_map[i], c->stage_rows)) != GNM_OK) return st;
+        if (!c->d_svrow[i] && (st = c->alloc(&c->d_svrow[i], c->stage_rows)) != GNM_OK) return st;
+        if (!c->d_bmoff[i] && (st = c->alloc(&c->d_bmoff[i], c->stage_rows)) != GNM_OK) return st;
+        if (!c->d_cs[i] && (st = c->alloc(&c->d_cs[i], ncs)) != GNM_OK) return st;
+        if (!c->h_map[i])
+            GNM_HIP(hipHostMalloc((void **)&c->h_map[i], c->stage_rows * sizeof(uint32_t), hipHostMallocDefault));
+        if (!c->h_svrow[i])
+            GNM_HIP(hipHostMalloc((void **)&c->h_svrow[i], c->stage_rows * sizeof(uint32_t), hipHostMallocDefault));
+        if (!c->h_bmoff[i])
+            GNM_HIP(hipHostMalloc((void **)&c->h_bmoff[i], c->stage_rows * sizeof(uint64_t), hipHostMallocDefault));
+        if (!c->h_cs[i]) GNM_HIP(hipHostMalloc((void **)&c->h_cs[i], ncs * sizeof(uint32_t), hipHostMallocDefault));
     }
     c->sparse_ready = true;
     return GNM_OK;
@@ -701,23 +706,31 @@ gnm_status gnm_score_rows(gnm_ctx *ctx, const uint8_t *rows, uint64_t row_stride
         GNM_HIP(hipSetDevice(ctx->device));
         gnm_ctx *c = ctx;
         const uint64_t bpv = c->bpv;
-        if (!c->h_stage[0]) {
+        if (!c->staging_ready) {
             // two pinned slots of ~256 MiB (at least one row) -- the reference's
             // PIPELINE_SLOTS = 2 double buffer (cuda_backend.rs:42, 1137-1147).
             // Copies run on their own stream (default priority), so the copy of
-            // one slot overlaps the scoring of the other.
+            // one slot overlaps the scoring of the other.  Each resource is
+            // made once: a call after a failed allocation resumes here.
             c->stage_pitch = (bpv + 63) / 64 * 64;  // 128-bit aligned rows for the staging loads
             c->stage_rows = std::min<uint64_t>(kSparseMaxSlotRows, std::max<uint64_t>(1, (256ull << 20) / c->stage_pitch));
-            GNM_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+            if (!c->copy_stream) GNM_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
             for (int i = 0; i < 2; ++i) {
-                GNM_HIP(hipHostMalloc((void **)&c->h_stage[i], c->stage_rows * c->stage_pitch, hipHostMallocDefault));
-                std::memset(c->h_stage[i], 0, c->stage_rows * c->stage_pitch);
-                gnm_status s2 = c->alloc(&c->d_stage[i], c->stage_rows * c->stage_pitch);
-                if (s2 != GNM_OK) return s2;
-                GNM_HIP(hipEventCreateWithFlags(&c->stage_free[i], hipEventDisableTiming));
-                GNM_HIP(hipEventCreateWithFlags(&c->stage_copied[i], hipEventDisableTiming));
-                GNM_HIP(hipEventRecord(c->stage_free[i], c->stream));
+                if (!c->h_stage[i]) {
+                    GNM_HIP(hipHostMalloc((void **)&c->h_stage[i], c->stage_rows * c->stage_pitch, hipHostMallocDefault));
+                    std::memset(c->h_stage[i], 0, c->stage_rows * c->stage_pitch);
+                }
+                if (!c->d_stage[i]) {
+                    gnm_status s2 = c->alloc(&c->d_stage[i], c->stage_rows * c->stage_pitch);
+                    if (s2 != GNM_OK) return s2;
+                }
+                if (!c->stage_free[i]) {
+                    GNM_HIP(hipEventCreateWithFlags(&c->stage_free[i], hipEventDisableTiming));
+                    GNM_HIP(hipEventRecord(c->stage_free[i], c->stream));
+                }
+                if (!c->stage_copied[i]) GNM_HIP(hipEventCreateWithFlags(&c->stage_copied[i], hipEventDisableTiming));
             }
+            c->staging_ready = true;
         }
         const bool use_sparse = c->sparse_fraction > 0.0;
         if (use_sparse && !c->sparse_ready) {
