@@ -470,7 +470,8 @@ def main():
     ap.add_argument("--compare", default="fp32,bf16x3",
                     help="comma-separated further paths timed on the same rows for the report ('' = none)")
     ap.add_argument("--variants", type=int, default=0, help="override the variant count")
-    ap.add_argument("--cpu-sample-variants", type=int, default=30000)
+    ap.add_argument("--cpu-sample-variants", type=int, default=300000,
+                    help="variants of the CPU-baseline sample (about 10 s on 16 host threads at 100k people)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="check a slice against the FP64 oracle")
     ap.add_argument("--no-unpack", action="store_true", help="skip the standalone unpack-stage measurement")
