@@ -213,3 +213,39 @@ def test_gradient_steps_reduce_the_loss():
     assert all(np.isfinite(losses))
     assert all(b <= a * (1 + 1e-9) for a, b in zip(losses, losses[1:]))
     assert losses[-1] < losses[0]
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "i8x4"])
+@pytest.mark.parametrize("N,V,K", [(1, 65, 1), (5, 1, 3), (33, 63, 2), (130, 200, 17)])
+def test_dense_forward_and_adjoint_tiny_shapes(orc, precision, N, V, K):
+    """One person, one variant, fewer than a k-step of rows, ragged bytes,
+    17 scores (a partial tile): forward within the limb bound and exact
+    missing counts; the adjoint within its bound (the int8 adjoint on rows
+    padded to 4 bytes)."""
+    _flush(f"tiny dense {precision} N={N} V={V} K={K}")
+    dos, rows, flip, W = _data(N, V, K, 100 + N + V + K)
+    S64, M64 = orc.dense_forward_fp64(rows, N, flip, W)
+    S, miss = ag.dense_forward(_dev(W), _dev(rows), _dev(flip), N, precision)
+    torch.cuda.synchronize()
+    S, miss = S.cpu().numpy(), miss.cpu().numpy()
+    assert np.array_equal(miss.astype(np.int64), M64.astype(np.int64))
+    scale = max(np.abs(S64).max(), 1e-30)
+    if precision == "fp32":
+        assert np.abs(S - S64).max() <= 1e-12 * scale
+    else:
+        G = _effect_dosage(dos, flip)
+        bound = _limb_bound(G, W.astype(np.float64), precision,
+                            I8_BLOCK if precision.startswith("i8") else BLOCK)
+        assert np.all(np.abs(S - S64) <= bound + 1e-12 * scale)
+    if precision == "fp32":
+        return   # the adjoint has split paths only
+    rng = np.random.default_rng(7)
+    R = rng.normal(0, 1, (N, K)).astype(np.float32)
+    dW64 = orc.adjoint_fp64(rows, N, flip, R.astype(np.float64))
+    arows = np.pad(rows, ((0, 0), (0, -rows.shape[1] % 4))) if precision.startswith("i8") else rows
+    dW = ag.dense_adjoint(_dev(R), _dev(arows), _dev(flip), V, precision)
+    torch.cuda.synchronize()
+    dW = dW.cpu().numpy().astype(np.float64)
+    Gt = _effect_dosage(dos, flip).T
+    bound = _limb_bound(Gt, R.astype(np.float64), precision, I8_BLOCK if precision.startswith("i8") else BLOCK)
+    assert np.all(np.abs(dW - dW64) <= bound + 2.0 ** -23 * np.abs(dW64) + 1e-30)
