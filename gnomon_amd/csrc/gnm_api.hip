@@ -137,6 +137,7 @@ struct gnm_ctx {
     uint64_t *h_bmoff[2] = {nullptr, nullptr}, *d_bmoff[2] = {nullptr, nullptr}; // [stage_rows] their records
     uint32_t *h_cs[2] = {nullptr, nullptr}, *d_cs[2] = {nullptr, nullptr};       // [stage_rows][nch+1]
     uint64_t st_dense = 0, st_sparse = 0, st_words = 0, st_bytes = 0;            // gnm_ingest_stats
+    uint32_t sparse_rest = 0;             // slots left to send packed without classifying
 
     template <class T>
     gnm_status alloc(T **out, size_t count) {
@@ -768,7 +769,12 @@ gnm_status gnm_score_rows(gnm_ctx *ctx, const uint8_t *rows, uint64_t row_stride
             const uint64_t ntask = (n + rows_per_task - 1) / rows_per_task;
             uint64_t nd = n, ne = 0, nsv = 0;
             std::vector<TaskOut> tout;
-            if (sparse_on) {
+            // after a slot in which every row went packed, the next 7 slots
+            // skip the classification (dense regions of the genome); every
+            // eighth slot looks again
+            const bool classify = sparse_on && c->sparse_rest == 0;
+            if (!classify && c->sparse_rest) --c->sparse_rest;
+            if (classify) {
                 // one pass: a task fills its bytes with its dense rows from the
                 // bottom and the records of its sparse rows from the top (rows
                 // of variants in no score are not sent at all)
@@ -815,6 +821,7 @@ gnm_status gnm_score_rows(gnm_ctx *ctx, const uint8_t *rows, uint64_t row_stride
                     ne += o.nwords;
                     nsv += o.svrow.size();
                 }
+                if (nd == n) c->sparse_rest = 7;
             }
             if (nd < n) {
                 uint64_t bytes = 0, s0 = 0;
@@ -855,9 +862,9 @@ gnm_status gnm_score_rows(gnm_ctx *ctx, const uint8_t *rows, uint64_t row_stride
                 if (st != GNM_OK) return st;
                 c->st_bytes += bytes + n * sizeof(uint32_t) + nsv * (nch + 4) * sizeof(uint32_t);   // + metadata
             } else {
-                // every row dense: the rows as they are (a sparse pass already
-                // wrote them in order)
-                if (!sparse_on) parallel_for(ntask, [&](uint64_t a, uint64_t b) {
+                // every row dense: the rows as they are (a classifying pass
+                // already wrote them in order)
+                if (!classify) parallel_for(ntask, [&](uint64_t a, uint64_t b) {
                     const uint64_t r0 = a * rows_per_task, r1 = std::min(n, b * rows_per_task);
                     if (row_stride == pitch) {
                         std::memcpy(dst + r0 * pitch, src + r0 * pitch, (r1 - r0) * pitch);
@@ -1004,6 +1011,7 @@ gnm_status gnm_set_sparse_ingest(gnm_ctx *ctx, double max_word_fraction) {
     if (!(max_word_fraction >= 0.0 && max_word_fraction <= 0.5))
         return fail(GNM_ERR_INVALID_ARG, "max_word_fraction must lie in [0, 1/2]");
     ctx->sparse_fraction = max_word_fraction;
+    ctx->sparse_rest = 0;
     return GNM_OK;
 }
 

@@ -133,7 +133,9 @@ gnm_status gnm_score_rows_device(gnm_ctx *ctx, const uint8_t *d_rows, uint64_t r
  * device rebuilds the packed row before scoring it; rows of variants in no
  * score are not sent.  Results are bit-identical to sending every row
  * packed.  max_word_fraction in [0, 1/2]; 0 sends every row packed.
- * Default 1/2.  Device-resident rows (gnm_score_rows_device) are read as
+ * Default 1/2.  After a staging slot in which every row went packed, the
+ * next 7 slots are sent packed without looking (dense regions cost no
+ * classification); every eighth slot looks again.  Device-resident rows (gnm_score_rows_device) are read as
  * they are. */
 gnm_status gnm_set_sparse_ingest(gnm_ctx *ctx, double max_word_fraction);
 /* Totals since gnm_create of the rows gnm_score_rows sent packed, the rows
