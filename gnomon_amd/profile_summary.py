@@ -6,6 +6,9 @@ Inputs are the directories rocprofv3 wrote with ``--output-format csv``:
   --write      a ``--pmc WRITE_SIZE`` run of the same command
   --cal-fetch / --cal-write   the same two counters over ``bench.py
                --calibrate-copy MIB`` (a plain elementwise copy of known size)
+  --pmc        (repeatable) any other ``--pmc`` run of the same command; every
+               counter it holds for the kernel is listed per launch, per wave
+               (when SQ_WAVES is among them) and per work unit (--units)
 
 The counters are scaled by what they read for the copy kernel: factor =
 bytes the copy moved / bytes the counter reported.  The summary ends with
@@ -50,6 +53,19 @@ def counter_per_dispatch(d, kernel):
     return statistics.mean(vals), len(vals)
 
 
+def counters_per_dispatch(d, kernel):
+    """{counter: mean per dispatch} for every counter of the kernel in a --pmc run."""
+    per = {}
+    for r in csv.DictReader(open(_one(d, "counter_collection.csv"))):
+        if kernel in r["Kernel_Name"]:
+            key = (r["Counter_Name"], r["Dispatch_Id"])
+            per[key] = per.get(key, 0.0) + float(r["Counter_Value"])   # summed over dimensions
+    out = {}
+    for (c, _), v in per.items():
+        out.setdefault(c, []).append(v)
+    return {c: statistics.mean(v) for c, v in out.items()}
+
+
 def calibration(d, cal_bytes):
     # the largest dispatches of the copy are the copy itself (torch splits a
     # tensor over 2^31 elements into equal launches)
@@ -75,6 +91,9 @@ def main():
     ap.add_argument("--cal-write")
     ap.add_argument("--cal-mib", type=int, default=2048)
     ap.add_argument("--alg-bytes", type=float, required=True)
+    ap.add_argument("--pmc", action="append", default=[])
+    ap.add_argument("--units", type=float, default=0.0, help="work units per launch, for per-unit counts")
+    ap.add_argument("--unit-name", default="unit")
     ap.add_argument("--command", action="append", default=[])
     ap.add_argument("-o", "--output", required=True)
     a = ap.parse_args()
@@ -113,6 +132,20 @@ def main():
                   f"| WRITE_SIZE | {write_kib:.1f} | {nw} | {write_b:.4e} |", "",
                   f"Traffic per launch {traffic:.4e} B against {a.alg_bytes:.4e} algorithmic bytes "
                   f"(ratio {traffic / a.alg_bytes:.3f}).", ""]
+    if a.pmc:
+        cnt = {}
+        for d in a.pmc:
+            cnt.update(counters_per_dispatch(d, a.kernel))
+        waves = cnt.get("SQ_WAVES")
+        lines += ["## Other counters (rocprofv3 --pmc, mean per launch)", "",
+                  f"| counter | per launch | per wave | per {a.unit_name} |", "|---|---|---|---|"]
+        for c in sorted(cnt):
+            pw = f"{cnt[c] / waves:.1f}" if waves else ""
+            pu = f"{cnt[c] / a.units:.2f}" if a.units else ""
+            lines.append(f"| {c} | {cnt[c]:.4e} | {pw} | {pu} |")
+        if a.units:
+            lines += ["", f"{a.unit_name}s per launch: {a.units:.4e}."]
+        lines.append("")
     lines += ["## Values read by bench.py", "", "```",
               f"kernel = {a.kernel}", f"workload = {a.workload}", f"precision = {a.precision}",
               f"kernel_avg_ms = {avg_ms:.6f}", f"algorithmic_bytes_per_launch = {a.alg_bytes:.0f}"]
