@@ -199,6 +199,18 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
     const uint32_t row_voff = (uint32_t)(4 * (uint64_t)rg * stride + ld_col);
 
     const uint64_t last_st = ((r_end - 1) / kI8Stage) * kI8Stage;
+    // Stages are numbered from st0, the split's first row aligned down to a
+    // batch; per-stage tests use these 32-bit indices (scalar compares, where
+    // 64-bit row compares take vector instructions):
+    //   nst      stages of the split
+    //   fast_lo, fast_hi   stages wholly inside [r_begin, r_end) (fast path)
+    //   last_i   the stage holding the split's last row
+    const uint64_t st0 = (r_begin / (D * kI8Stage)) * (D * kI8Stage);
+    const bool nonempty = r_begin < r_end;
+    const uint32_t nst = nonempty ? (uint32_t)((r_end - st0 + kI8Stage - 1) / kI8Stage) : 0u;
+    const uint32_t fast_lo = (uint32_t)((r_begin - st0 + kI8Stage - 1) / kI8Stage);
+    const uint32_t fast_hi = (IDENT && span_full && nonempty) ? (uint32_t)((r_end - st0) / kI8Stage) : 0u;
+    const uint32_t last_i = nonempty ? (uint32_t)((last_st - st0) / kI8Stage) : 0u;
     // B fragments: wave w fetches tiles w, w + WAVES, ... of the sets' kBT
     // tiles by LDS DMA (lane l's 16 bytes land at btile[d][64 j + l]); tiles
     // past tile_end are fetched clamped to the last tile and their products
@@ -210,9 +222,12 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
         const uint32_t jc = j < tile_end ? j : tile_end - 1;
         boff[m] = (jc * 64 + lane) * 16;
     }
-    auto fetch_b = [&](uint32_t par, int d, uint64_t st) __attribute__((always_inline)) {
+    // fragment block of stage index si (clamped to last_i by the caller)
+    const u32x4_t *__restrict__ bfrag_st0 = bfrag + (st0 / kI8Stage) * tiles_total * 64;
+    const uint32_t bstep = tiles_total * 64;
+    auto fetch_b = [&](uint32_t par, int d, uint32_t si) __attribute__((always_inline)) {
         const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
-            (void *)(bfrag + (st / kI8Stage) * tiles_total * 64), 0, 0x7FFFFFFF, kI8RsrcWord3);
+            (void *)(bfrag_st0 + (uint64_t)si * bstep), 0, 0x7FFFFFFF, kI8RsrcWord3);
 #pragma unroll
         for (int m = 0; m < kBW; ++m) {
             const int j = wave + m * WAVES;
@@ -232,8 +247,9 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
         }
         return x;
     };
-    auto issue = [&](uint32_t par, int d, uint64_t st_req) __attribute__((always_inline)) {
-        if (IDENT && span_full && st_req >= r_begin && st_req + kI8Stage <= r_end) {
+    auto issue = [&](uint32_t par, int d, uint32_t si) __attribute__((always_inline)) {
+        const uint64_t st_req = st0 + (uint64_t)si * kI8Stage;
+        if (si >= fast_lo && si < fast_hi) {   // IDENT && span_full, stage inside the split
             // buffer loads: a uniform descriptor per stage, one 32-bit lane
             // offset, and the row step as the scalar offset
             const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
@@ -445,25 +461,23 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
     // row loads and B-fragment DMA are issued before the previous batch is
     // computed, so their latency overlaps the MFMAs; a batch is stored to
     // LDS only once its loads have landed.
-    const uint64_t st0 = (r_begin / (D * kI8Stage)) * (D * kI8Stage);
-    const uint64_t nst = (r_end - st0 + kI8Stage - 1) / kI8Stage;
-    auto issue_batch = [&](uint32_t par, uint64_t i) __attribute__((always_inline)) {
+    auto issue_batch = [&](uint32_t par, uint32_t i) __attribute__((always_inline)) {
         // stages past the split are loaded clamped and never stored
         if (!loader) return;
         [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
-            (issue(par, I, st0 + (i + I) * kI8Stage), ...);
+            (issue(par, I, i + I), ...);
         }(std::make_integer_sequence<int, D>{});
     };
-    auto fetch_batch = [&](uint32_t par, uint64_t i) __attribute__((always_inline)) {
+    auto fetch_batch = [&](uint32_t par, uint32_t i) __attribute__((always_inline)) {
         [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
-            (fetch_b(par, I, (st0 + (i + I) * kI8Stage) < last_st ? st0 + (i + I) * kI8Stage : last_st), ...);
+            (fetch_b(par, I, i + I < last_i ? i + I : last_i), ...);
         }(std::make_integer_sequence<int, D>{});
     };
     auto main_loop = [&](auto nwc) __attribute__((always_inline)) {
         Acc acc;   // one accumulator set per copy of the loop
         zero_acc(acc);
         uint32_t par = 0;
-        uint64_t i0 = 0;
+        uint32_t i0 = 0;
         issue_batch(par, i0);
         fetch_batch(par, i0);
         // Outer loop over scale blocks, inner loop over batches of the block:
@@ -472,9 +486,9 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
         // stay in accumulation registers.
         bool first = true;
         do {   // nst > 0
-            const uint64_t blk = (st0 + i0 * kI8Stage) / kI8FlushRows;
+            const uint64_t blk = (st0 + (uint64_t)i0 * kI8Stage) / kI8FlushRows;
             const uint64_t blk_end = ((blk + 1) * kI8FlushRows - st0) / kI8Stage;   // a multiple of D
-            const uint64_t i_end = blk_end < nst ? blk_end : nst;
+            const uint32_t i_end = blk_end < nst ? (uint32_t)blk_end : nst;
             do {   // i0 < i_end on entry: the block holds stage i0
                 if (loader) {
                     [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
