@@ -2,7 +2,7 @@
 
 The kernel has two shape parameters (gnm_set_launch_config):
   * tiles_per_pass -- 16-column weight/count tiles one workgroup accumulates
-    (3, 4, 6 or 8).  More tiles per workgroup decode each .bed stage once for
+    (3, 4, 5, 6 or 8).  More tiles per workgroup decode each .bed stage once for
     more columns but hold more accumulators (fewer waves per SIMD); fewer
     tiles give more workgroups per CU and more decode work.  All tile groups
     run in one launch either way.
@@ -22,7 +22,7 @@ from __future__ import annotations
 import json
 import os
 
-CANDIDATES = [(3, 4), (3, 2), (4, 4), (4, 2), (6, 4), (6, 2), (8, 4), (8, 2), (18, 2)]
+CANDIDATES = [(3, 4), (3, 2), (4, 4), (4, 2), (5, 4), (5, 2), (6, 4), (6, 2), (8, 4), (8, 2), (18, 2)]
 
 _memory: dict[str, tuple[int, int]] = {}
 

@@ -994,9 +994,9 @@ gnm_status gnm_unpack_device(const uint8_t *d_rows, uint64_t bytes_per_variant, 
 
 gnm_status gnm_set_launch_config(gnm_ctx *ctx, uint32_t tiles_per_pass, uint32_t batch_stages) {
     if (!ctx) return fail(GNM_ERR_INVALID_ARG, "ctx is NULL");
-    if (tiles_per_pass != 0 && tiles_per_pass != 3 && tiles_per_pass != 4 && tiles_per_pass != 6 &&
-        tiles_per_pass != 8 && tiles_per_pass != 18)
-        return fail(GNM_ERR_INVALID_ARG, "tiles_per_pass must be 0, 3, 4, 6, 8 or 18");
+    if (tiles_per_pass != 0 && tiles_per_pass != 3 && tiles_per_pass != 4 && tiles_per_pass != 5 &&
+        tiles_per_pass != 6 && tiles_per_pass != 8 && tiles_per_pass != 18)
+        return fail(GNM_ERR_INVALID_ARG, "tiles_per_pass must be 0, 3, 4, 5, 6, 8 or 18");
     if (tiles_per_pass == 18 && batch_stages == 4)
         return fail(GNM_ERR_INVALID_ARG, "tiles_per_pass 18 runs two-stage batches only");
     if (batch_stages != 0 && batch_stages != 2 && batch_stages != 4)

@@ -532,10 +532,10 @@ static uint64_t i8_pass_cost(uint32_t ntiles, uint32_t tmax) {
     return groups * (240u + 64u * tmax);
 }
 
-// default tiles per pass: the cheapest of 3, 4, 6 and 8 by the cost model
+// default tiles per pass: the cheapest of 3, 4, 5, 6 and 8 by the cost model
 uint32_t i8_default_tiles_per_pass(uint32_t ntiles) {
     uint32_t tmax = 3;
-    for (uint32_t c : {4u, 6u, 8u})
+    for (uint32_t c : {4u, 5u, 6u, 8u})
         if (i8_pass_cost(ntiles, c) < i8_pass_cost(ntiles, tmax)) tmax = c;
     return tmax;
 }
@@ -566,11 +566,11 @@ hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t
     rows_per_split = (rows_per_split + kI8Stage - 1) / kI8Stage * kI8Stage;
     const uint32_t ntiles = tile_end - tile_begin;
     // tiles per workgroup pass: every group computes tmax tile slots, so the
-    // default takes the cheapest of 3, 4, 6 and 8 by the cost model above;
+    // default takes the cheapest of 3, 4, 5, 6 and 8 by the cost model above;
     // 18 is two sets of 9-tile groups over the same people (one row read
     // for both)
     const uint32_t tp = tiles_per_pass ? tiles_per_pass : i8_default_tiles_per_pass(ntiles);
-    if (tp != 3 && tp != 4 && tp != 6 && tp != 8 && tp != 18) return hipErrorInvalidValue;
+    if (tp != 3 && tp != 4 && tp != 5 && tp != 6 && tp != 8 && tp != 18) return hipErrorInvalidValue;
     // stages per batch by default: two for 3-tile groups (the smaller
     // register ring keeps four waves per SIMD), 4 and 8 and the two-set
     // shape; four for 6-tile groups (more loads in flight; 6% faster at
@@ -606,6 +606,7 @@ hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t
     } while (0)
     if (tp == 3) GNM_LAUNCH_I8_D(3, 8);
     else if (tp == 4) GNM_LAUNCH_I8_D(4, 4);
+    else if (tp == 5) GNM_LAUNCH_I8_D(5, 4);
     else if (tp == 6) GNM_LAUNCH_I8_D(6, 4);
     else if (tp == 8) GNM_LAUNCH_I8_D(8, 4);
     else GNM_LAUNCH_I8_VI(9, 8, 2, 2);

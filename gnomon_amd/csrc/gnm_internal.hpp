@@ -91,7 +91,7 @@ hipError_t launch_score_mfma(const DevicePlan &plan, const uint8_t *rows, uint64
 // Split-int8 scoring (gnm_i8.hip); rows must be 4-byte aligned with a
 // 4-byte multiple stride below 2^25 for the dword staging loads (else byte
 // loads).  One launch covers every tile group of [tile_begin, tile_end)
-// (groups of tiles_per_pass tiles: 0 = default, else 3, 4, 6 or 8, or 18
+// (groups of tiles_per_pass tiles: 0 = default, else 3, 4, 5, 6 or 8, or 18
 // for two sets of 9-tile groups over the same people;
 // batch_stages: 0 = default, else 2 or 4 stages whose loads are issued
 // together).

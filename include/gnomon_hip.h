@@ -227,7 +227,7 @@ gnm_status gnm_synth_rows_device(uint64_t seed, uint32_t p_lo24, uint32_t p_hi24
 /* Launch shape of the split-int8 scoring kernel on this context (tuning; no
  * reference counterpart, used by the project's autotuner):
  *   tiles_per_pass: 16-column weight/count tiles each workgroup accumulates
- *                   (0 = default, 3, 4, 6 or 8; 18 = two sets of waves with a
+ *                   (0 = default, 3, 4, 5, 6 or 8; 18 = two sets of waves with a
  *                   9-tile group each over the same people, one row read for
  *                   both, two-stage batches only; all tile groups of a block
  *                   still run in one launch);

@@ -319,7 +319,7 @@ def test_int8_scale_blocks_and_launch_shapes(orc, precision):
     csr = csr_from_dense(W, rng.random((V, K)) < 0.5, present)
     S64, C64 = orc.score_fp64(rows, csr, N)
     first = None
-    for tp, bs in [(0, 0), (3, 2), (3, 4), (4, 4), (6, 2), (6, 4), (8, 2), (8, 4), (18, 2)]:
+    for tp, bs in [(0, 0), (3, 2), (3, 4), (4, 4), (5, 2), (5, 4), (6, 2), (6, 4), (8, 2), (8, 4), (18, 2)]:
         _flush(f"  tiles_per_pass={tp} batch_stages={bs}")
         with ScoringContext(csr, N, precision=precision, force=True) as ctx:
             ctx.set_launch_config(tp, bs)
