@@ -488,8 +488,8 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}); the path's sharding is "
-              "described in DESIGN.md and not measured here", file=sys.stderr)
+        print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}): this build scores on one "
+              "device, and splitting people over several GPUs is a design, not a measured path", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
