@@ -23,7 +23,8 @@ constexpr int flush_rows(bool int8) { return int8 ? kI8FlushRows : kFlushRows; }
 constexpr int kLimbBits = 8;
 constexpr int kFp32ScoreChunk = 8;     // scores per pass of the FP32 kernel
 constexpr int kFp32MaxClasses = 4;     // membership classes counted in-kernel by the FP32 kernel
-constexpr int kMfmaMaxTiles = 8;       // 16-column tiles per pass of the MFMA kernel
+constexpr int kMfmaMaxTiles = 4;       // 16-column tiles per pass of the MFMA kernel (two waves per SIMD;
+                                        // 8-tile passes fit one and took 98 ms against 76 ms at 100k x 1M x 64)
 
 // Signed base-256 digits of an integer with |Q| <= 2^(8L) - 2^(8(L-1)):
 // low digits in [-128, 127], top digit in [-255, 255]; all exact in BF16.

@@ -867,8 +867,7 @@ hipError_t launch_score_mfma(const DevicePlan &plan, const uint8_t *rows, uint64
             if (identity) GNM_LAUNCH_MFMA(TM, WV, DD, false, true); else GNM_LAUNCH_MFMA(TM, WV, DD, false, false); \
         }                                                                                                    \
     } while (0)
-        if (nt <= 4) GNM_LAUNCH_MFMA_V(4, 4, 4);
-        else GNM_LAUNCH_MFMA_V(kMfmaMaxTiles, 4, 2);
+        GNM_LAUNCH_MFMA_V(kMfmaMaxTiles, 4, 4);
 #undef GNM_LAUNCH_MFMA_V
 #undef GNM_LAUNCH_MFMA
         hipError_t e = hipGetLastError();
