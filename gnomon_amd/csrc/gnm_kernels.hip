@@ -825,7 +825,8 @@ static uint32_t splits_for(uint64_t wgs_per_split) {
     return (uint32_t)(s < 8 ? 8 : (s > 64 ? 64 : s));
 }
 
-static int mfma_waves(uint32_t) { return 4; }
+// 8 waves: each workgroup reads 128 contiguous bytes of a row (whole lines)
+static int mfma_waves(uint32_t) { return 8; }
 
 uint32_t mfma_splits(uint64_t num_people, uint32_t tiles) {
     const uint32_t nt = tiles < (uint32_t)kMfmaMaxTiles ? tiles : (uint32_t)kMfmaMaxTiles;
@@ -867,7 +868,7 @@ hipError_t launch_score_mfma(const DevicePlan &plan, const uint8_t *rows, uint64
             if (identity) GNM_LAUNCH_MFMA(TM, WV, DD, false, true); else GNM_LAUNCH_MFMA(TM, WV, DD, false, false); \
         }                                                                                                    \
     } while (0)
-        GNM_LAUNCH_MFMA_V(kMfmaMaxTiles, 4, 4);
+        GNM_LAUNCH_MFMA_V(kMfmaMaxTiles, 8, 4);
 #undef GNM_LAUNCH_MFMA_V
 #undef GNM_LAUNCH_MFMA
         hipError_t e = hipGetLastError();
