@@ -260,6 +260,8 @@ __global__ __launch_bounds__(256, 2) void k_adjoint(const uint8_t *__restrict__ 
     const uint64_t bpv = (N + 3) / 4;
     const uint8_t *__restrict__ row = rows + (vok ? v : 0) * stride;
     const bool fl = vok && flip && flip[v];
+    // 128-bit row loads need 16-byte aligned rows (base and stride)
+    const bool row16 = ((reinterpret_cast<uintptr_t>(rows) | stride) & 15u) == 0;
     const uint64_t nchunks = (N + 255) / 256;
     const uint64_t ncolp = (uint64_t)L * Kp;
 
@@ -280,20 +282,30 @@ __global__ __launch_bounds__(256, 2) void k_adjoint(const uint8_t *__restrict__ 
             // 16 bytes of this lane's row: people 256ch + 64lg .. +64
             const uint64_t off = ch * 64 + 16 * lg;
             uint32_t w[4];
+            if (vok && row16 && off + 16 <= bpv) {
+                // whole 16 bytes inside the row: one 128-bit load
+                const uint4 q = *(const uint4 *)(row + off);
+                w[0] = q.x; w[1] = q.y; w[2] = q.z; w[3] = q.w;
+                if (fl) {
 #pragma unroll
-            for (int d = 0; d < 4; ++d) {
-                uint32_t x = 0x55555555u;  // missing: contributes nothing
-                if (vok) {
-                    x = 0;
-#pragma unroll
-                    for (int bb = 0; bb < 4; ++bb) {
-                        const uint64_t o = off + 4 * d + bb;
-                        const uint32_t byte = o < bpv ? row[o] : 0x55u;
-                        x |= byte << (8 * bb);
-                    }
-                    if (fl) x = d_flip_word(x);
+                    for (int d = 0; d < 4; ++d) w[d] = d_flip_word(w[d]);
                 }
-                w[d] = x;
+            } else {
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    uint32_t x = 0x55555555u;  // missing: contributes nothing
+                    if (vok) {
+                        x = 0;
+#pragma unroll
+                        for (int bb = 0; bb < 4; ++bb) {
+                            const uint64_t o = off + 4 * d + bb;
+                            const uint32_t byte = o < bpv ? row[o] : 0x55u;
+                            x |= byte << (8 * bb);
+                        }
+                        if (fl) x = d_flip_word(x);
+                    }
+                    w[d] = x;
+                }
             }
 #pragma unroll
             for (int s8 = 0; s8 < 8; ++s8) {
