@@ -334,6 +334,27 @@ def test_int8_scale_blocks_and_launch_shapes(orc, precision):
             assert np.array_equal(S, first[0]), f"shape {tp}x{bs} differs"
 
 
+def test_launch_config_rejects_unknown_shapes(orc):
+    """Tiles per pass outside {0, 3, 4, 5, 6, 8, 18}, batch depths outside
+    {0, 2, 4}, and the two-set shape with four-stage batches are refused
+    with an invalid-argument error; the context still scores afterwards."""
+    from gnomon_amd._lib import GnmError
+    rng = np.random.default_rng(41)
+    N, V, K = 300, 2_000, 3
+    rows = orc.synth_rows(8, N, 3, V, int(0.02 * 2**24), int(0.5 * 2**24), 1409)
+    csr = csr_from_dense(rng.normal(0, 0.01, (V, K)).astype(np.float32), rng.random((V, K)) < 0.5,
+                         np.ones((V, K), bool))
+    S64, C64 = orc.score_fp64(rows, csr, N)
+    with ScoringContext(csr, N, precision="i8x4", force=True) as ctx:
+        for tp, bs in [(1, 0), (2, 2), (7, 2), (9, 2), (16, 2), (18, 4), (5, 3), (3, 8)]:
+            with pytest.raises(GnmError):
+                ctx.set_launch_config(tp, bs)
+        ctx.set_launch_config(5, 2)
+        ctx.score_rows(rows, 0)
+        S, C = ctx.finish()
+    _check_scores(S, C, S64, C64, "i8x4")
+
+
 def test_int8_two_set_shape_many_scores(orc):
     """The two-set shape (two 9-tile groups over the same people, one row read
     for both) with 40 scores: 10 weight tiles and 3 count tiles, so the second
