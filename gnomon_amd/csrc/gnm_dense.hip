@@ -320,12 +320,17 @@ __global__ __launch_bounds__(256, 2) void k_adjoint(const uint8_t *__restrict__ 
                 }
                 const uint64_t ks = ch * 8 + s8;
                 const u32x4_t *bf = (const u32x4_t *)frag + (ks * tiles) * 64 + lane;
+                // the k-step's fragments load together, without branches
+                // (slots past the last tile read the last tile again and
+                // their products are never used), then feed the MFMAs
+                u32x4_t bv[TMAX];
+#pragma unroll
+                for (int j = 0; j < TMAX; ++j) bv[j] = bf[(uint64_t)((uint32_t)j < tiles ? j : tiles - 1) * 64];
 #pragma unroll
                 for (int j = 0; j < TMAX; ++j) {
                     if ((uint32_t)j < tiles) {
                         DFrag B;
-                        const u32x4_t bv = bf[(uint64_t)j * 64];
-                        B.u[0] = bv[0]; B.u[1] = bv[1]; B.u[2] = bv[2]; B.u[3] = bv[3];
+                        B.u[0] = bv[j][0]; B.u[1] = bv[j][1]; B.u[2] = bv[j][2]; B.u[3] = bv[j][3];
                         acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.v, B.v, acc[j], 0, 0, 0);
                     }
                 }
@@ -366,6 +371,7 @@ hipError_t launch_adjoint(const uint8_t *rows, uint64_t stride, uint64_t V, uint
                           uint32_t L, float *dW, hipStream_t stream) {
     const uint64_t wgs = (V + 63) / 64;
     if (wgs == 0) return hipSuccess;
+    if (tiles == 0) return hipErrorInvalidValue;   // the kernel reads tile min(j, tiles - 1)
     if (tiles <= 4) {
         hipLaunchKernelGGL(k_adjoint<4>, dim3((unsigned)wgs), dim3(256), 0, stream, rows, stride, V, N, flip, frag,
                            scales, tiles, K, Kp, L, dW);
