@@ -307,8 +307,27 @@ __global__ __launch_bounds__(256, 2) void k_adjoint(const uint8_t *__restrict__ 
                     w[d] = x;
                 }
             }
+            // a k-step's fragments load together, without branches (slots
+            // past the last tile read the last tile again and their products
+            // are never used); the next k-step's are in flight while this
+            // one's MFMAs run
+            auto load_b = [&](u32x4_t (&bv)[TMAX], uint64_t ks) __attribute__((always_inline)) {
+                const u32x4_t *bf = (const u32x4_t *)frag + (ks * tiles) * 64 + lane;
+#pragma unroll
+                for (int j = 0; j < TMAX; ++j) bv[j] = bf[(uint64_t)((uint32_t)j < tiles ? j : tiles - 1) * 64];
+            };
+            // (prefetch only where registers allow: the 12-tile instance
+            // would spill)
+            constexpr bool kPrefetch = TMAX <= 4;
+            u32x4_t bvb[kPrefetch ? 2 : 1][TMAX];
+            if constexpr (kPrefetch) load_b(bvb[0], ch * 8);
 #pragma unroll
             for (int s8 = 0; s8 < 8; ++s8) {
+                if constexpr (kPrefetch) {
+                    if (s8 < 7) load_b(bvb[(s8 + 1) & 1], ch * 8 + s8 + 1);
+                } else {
+                    load_b(bvb[0], ch * 8 + s8);
+                }
                 const uint32_t b0 = (w[s8 >> 1] >> (16 * (s8 & 1))) & 0xFFu;
                 const uint32_t b1 = (w[s8 >> 1] >> (16 * (s8 & 1) + 8)) & 0xFFu;
                 const uint32_t pp = (b0 | (b1 << 16)) * 0x0101u;
@@ -318,14 +337,7 @@ __global__ __launch_bounds__(256, 2) void k_adjoint(const uint8_t *__restrict__ 
                     const uint32_t x = ((pp >> (2 * q)) & 0x03030303u) | 0x04000400u;
                     A.u[q] = __builtin_amdgcn_perm(0x403F0000u, 0x00800000u, x);
                 }
-                const uint64_t ks = ch * 8 + s8;
-                const u32x4_t *bf = (const u32x4_t *)frag + (ks * tiles) * 64 + lane;
-                // the k-step's fragments load together, without branches
-                // (slots past the last tile read the last tile again and
-                // their products are never used), then feed the MFMAs
-                u32x4_t bv[TMAX];
-#pragma unroll
-                for (int j = 0; j < TMAX; ++j) bv[j] = bf[(uint64_t)((uint32_t)j < tiles ? j : tiles - 1) * 64];
+                const u32x4_t (&bv)[TMAX] = bvb[kPrefetch ? (s8 & 1) : 0];
 #pragma unroll
                 for (int j = 0; j < TMAX; ++j) {
                     if ((uint32_t)j < tiles) {
