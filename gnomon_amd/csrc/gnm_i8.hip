@@ -543,14 +543,17 @@ uint32_t i8_default_tiles_per_pass(uint32_t ntiles) {
 // Variant splits of the int8 kernel: enough that all tile groups together
 // launch about 12k workgroups (~48 per CU, so that three stay resident per
 // CU through the tail and many row strips are read at once), a multiple of
-// 8 (whole splits per XCD), at most 64 (the FP64 partials grow with it).
+// 8 (whole splits per XCD), at most 32: the FP64 partials and their
+// reduction grow with the split count (at 50k people x 16 scores, 64 splits
+// wrote and reduced 1.6 GB of partials; 32 splits ran the forward 9% faster,
+// 16 no faster than 32).
 uint32_t i8_splits(uint64_t num_people, uint32_t ntiles) {
     const uint32_t tmax = i8_default_tiles_per_pass(ntiles ? ntiles : 1);
     const uint64_t groups = ((ntiles ? ntiles : 1) + tmax - 1) / tmax;
     const uint64_t per = ((num_people + 255) / 256) * groups;
     uint64_t s = (12288 + per - 1) / per;
     s = (s + 7) / 8 * 8;
-    return (uint32_t)(s < 8 ? 8 : (s > 64 ? 64 : s));
+    return (uint32_t)(s < 8 ? 8 : (s > 32 ? 32 : s));
 }
 
 hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t stride, uint64_t bpv, uint64_t v0,
