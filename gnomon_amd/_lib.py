@@ -27,7 +27,7 @@ STATUS_NAMES = {
     GNM_ERR_HIP: "HIP_ERROR", GNM_ERR_INTERNAL: "INTERNAL", GNM_ERR_UNSUPPORTED: "UNSUPPORTED",
 }
 
-PRECISION = {"fp32": 0, "bf16x2": 1, "bf16x3": 2, "auto": 3, "i8x3": 4, "i8x4": 5}
+PRECISION = {"fp32": 0, "bf16x2": 1, "bf16x3": 2, "auto": 3, "i8x3": 4, "i8x4": 5, "bf16x4": 6}
 
 # Every entry point declared in include/gnomon_hip.h.
 EXPORTS = (

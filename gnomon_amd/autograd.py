@@ -22,7 +22,7 @@ import torch
 
 from . import _lib
 
-_PREC = {k: _lib.PRECISION[k] for k in ("fp32", "bf16x2", "bf16x3", "i8x3", "i8x4")}
+_PREC = {k: _lib.PRECISION[k] for k in ("fp32", "bf16x2", "bf16x3", "bf16x4", "i8x3", "i8x4", "auto")}
 
 
 def _ptr(t):
@@ -42,10 +42,38 @@ def _check_bed(bed, num_people, V):
         raise ValueError("bed rows are shorter than ceil(num_people / 4) bytes")
 
 
-def dense_forward(W, bed, flip, num_people, precision="bf16x3"):
+def _check_operand(name, t, bed, rows, cols=None):
+    """t must be a 2-D (or, with cols None, 1-D) tensor on bed's device with
+    the given shape; checked before any kernel is launched."""
+    if not isinstance(t, torch.Tensor) or t.device != bed.device:
+        raise ValueError(f"{name} must be a tensor on {bed.device}")
+    shape = (rows,) if cols is None else (rows, cols)
+    if tuple(t.shape) != shape:
+        raise ValueError(f"{name} must have shape {shape}, got {tuple(t.shape)}")
+
+
+def _check_flip(flip, bed, V):
+    if flip is None:
+        return
+    _check_operand("flip", flip, bed, V)
+    if flip.dtype not in (torch.uint8, torch.bool) or not flip.is_contiguous():
+        raise ValueError("flip must be a contiguous uint8 or bool tensor (1 = effect allele is A1)")
+
+
+def _check_precision(precision):
+    if precision not in _PREC:
+        raise ValueError(f"unknown precision {precision!r}; one of {sorted(_PREC)}")
+
+
+def dense_forward(W, bed, flip, num_people, precision="auto"):
     """Scores [N, K] float64 and per-person missing counts [N] int32."""
+    _check_precision(precision)
+    if not isinstance(W, torch.Tensor) or W.dim() != 2:
+        raise ValueError("W must be a 2-D tensor [variants, scores]")
     V, K = W.shape
     _check_bed(bed, num_people, V)
+    _check_operand("W", W, bed, V, K)
+    _check_flip(flip, bed, V)
     W = W.detach().to(torch.float32).contiguous()
     S = torch.empty(num_people, K, dtype=torch.float64, device=W.device)
     miss = torch.empty(num_people, dtype=torch.int32, device=W.device)
@@ -55,10 +83,15 @@ def dense_forward(W, bed, flip, num_people, precision="bf16x3"):
     return S, miss
 
 
-def dense_adjoint(R, bed, flip, num_variants, precision="bf16x3"):
+def dense_adjoint(R, bed, flip, num_variants, precision="auto"):
     """dW [V, K] float32 = G^T R for R [N, K]."""
+    _check_precision(precision)
+    if not isinstance(R, torch.Tensor) or R.dim() != 2:
+        raise ValueError("R must be a 2-D tensor [people, scores]")
     N, K = R.shape
     _check_bed(bed, N, num_variants)
+    _check_operand("R", R, bed, N, K)
+    _check_flip(flip, bed, num_variants)
     R = R.detach().to(torch.float32).contiguous()
     dW = torch.empty(num_variants, K, dtype=torch.float32, device=R.device)
     _lib.check(_lib.lib().gnm_dense_adjoint_device(
@@ -71,7 +104,10 @@ def _adjoint_precision(precision, bed):
     """The adjoint's arithmetic for a forward precision: split-int8 for the
     int8 paths when the rows are 4-byte aligned with a row stride that is a
     multiple of 4 (what the int8 kernel's dword loads need), else split-BF16
-    (bf16x3 unless bf16x2 is asked for)."""
+    (bf16x3 unless bf16x2 is asked for); "auto" is resolved by the library
+    (split-int8 where the rows allow it and its self-test passed)."""
+    if precision == "auto":
+        return "auto"
     if precision in ("i8x4", "i8x3"):
         if bed.data_ptr() % 4 == 0 and bed.shape[1] % 4 == 0:
             return precision
@@ -85,13 +121,14 @@ class PolygenicScore(torch.autograd.Function):
     W: [V, K] float32 CUDA tensor; bed: uint8 CUDA tensor [V, row_stride]
     (packed SNP-major rows); flip: uint8 CUDA tensor [V] (1 = effect allele
     is A1) or None.  Returns S [N, K] float64.  precision selects the forward
-    arithmetic ("bf16x3", "bf16x2", "i8x4", "i8x3" or "fp32"); the adjoint
+    arithmetic ("auto", "bf16x4", "bf16x3", "bf16x2", "i8x4", "i8x3" or
+    "fp32"; "auto" is split-int8 after the library's self-test); the adjoint
     runs split-int8 with the int8 forwards, else split-BF16 (bf16x3 unless
     "bf16x2" is asked for).
     """
 
     @staticmethod
-    def forward(ctx, W, bed, flip, num_people, precision="bf16x3"):
+    def forward(ctx, W, bed, flip, num_people, precision="auto"):
         S, miss = dense_forward(W, bed, flip, num_people, precision)
         ctx.save_for_backward(bed, flip) if flip is not None else ctx.save_for_backward(bed)
         ctx.has_flip = flip is not None
@@ -108,5 +145,5 @@ class PolygenicScore(torch.autograd.Function):
         return dW.to(ctx.w_dtype), None, None, None, None
 
 
-def polygenic_score(W, bed, flip, num_people, precision="bf16x3"):
+def polygenic_score(W, bed, flip, num_people, precision="auto"):
     return PolygenicScore.apply(W, bed, flip, num_people, precision)

@@ -179,16 +179,22 @@ gnm_status validate(const gnm_problem *p) {
     for (uint64_t v = 0; v < p->num_variants; ++v)
         if (p->sparse_row_offsets[v + 1] < p->sparse_row_offsets[v])
             return fail(GNM_ERR_INVALID_ARG, "sparse_row_offsets must be non-decreasing");
-    for (uint64_t e = 0; e < p->nnz; ++e)
+    for (uint64_t e = 0; e < p->nnz; ++e) {
         if (p->sparse_score_columns[e] >= p->num_scores)
             return fail(GNM_ERR_INVALID_ARG, "CSR score column out of range");
+        // non-finite weights are refused, as the reference refuses its split
+        // path for them (score/cuda_backend.rs:1178-1181)
+        if (!std::isfinite(p->sparse_weights[e]) || !std::isfinite(p->sparse_missing_corrections[e]))
+            return fail(GNM_ERR_INVALID_ARG, "CSR weight or missing correction is not finite (entry " +
+                                                 std::to_string(e) + ")");
+    }
     if (!p->output_idx_to_fam_idx && p->num_people != p->total_people_in_fam)
         return fail(GNM_ERR_INVALID_ARG, "output_idx_to_fam_idx is required when num_people < total_people_in_fam");
     if (p->output_idx_to_fam_idx)
         for (uint64_t i = 0; i < p->num_people; ++i)
             if (p->output_idx_to_fam_idx[i] >= p->total_people_in_fam)
                 return fail(GNM_ERR_INVALID_ARG, "output_idx_to_fam_idx entry out of range");
-    if (p->precision < GNM_PRECISION_FP32 || p->precision > GNM_PRECISION_I8X4)
+    if (p->precision < GNM_PRECISION_FP32 || p->precision > GNM_PRECISION_BF16X4)
         return fail(GNM_ERR_INVALID_ARG, "unknown precision");
     for (int i = 0; i < 4; ++i)
         if (p->reserved[i]) return fail(GNM_ERR_INVALID_ARG, "reserved fields must be zero");
@@ -203,6 +209,7 @@ inline uint32_t limbs_of(int precision) {
     switch (precision) {
         case GNM_PRECISION_BF16X2: return 2;
         case GNM_PRECISION_BF16X3: return 3;
+        case GNM_PRECISION_BF16X4: return 4;
         case GNM_PRECISION_I8X3: return 3;
         case GNM_PRECISION_I8X4: return 4;
         default: return 1;
@@ -278,7 +285,7 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
     c->bpv = p->bytes_per_variant;
     c->K = p->num_scores;
     c->V = p->num_variants;
-    c->precision = p->precision == GNM_PRECISION_AUTO ? GNM_PRECISION_BF16X3 : p->precision;
+    c->precision = p->precision;   // AUTO is resolved by the caller (create_impl)
     c->device = p->device < 0 ? 0 : p->device;
     c->limbs = limbs_of(c->precision);
     const bool int8 = is_int8(c->precision);
@@ -624,6 +631,93 @@ gnm_status check_block(gnm_ctx *c, uint64_t first_variant, uint64_t num_rows, ui
     return GNM_OK;
 }
 
+// Known-answer check of the split-int8 path on one device, the counterpart
+// of the reference's split-path self-test calibrate_fp16
+// (score/cuda_backend.rs:2424-2494), with its inputs: 64 people x 64 variants
+// x 64 scores, dosage (64 v + p) mod 3, weight ((7 i mod 101) - 50) / 13 for
+// i = 64 v + k.  Passes when the counts are exact (all zero) and every score
+// is within the limb bound sum_v d_pv * max_v |w_vk| / qmax of the exact sum.
+bool split_self_test(int device) {
+    constexpr uint64_t D = 64;
+    std::vector<float> w(D * D), co(D * D, 0.0f);
+    std::vector<uint32_t> cols(D * D);
+    std::vector<uint64_t> off(D + 1);
+    for (uint64_t v = 0; v < D; ++v) {
+        off[v] = v * D;
+        for (uint64_t k = 0; k < D; ++k) {
+            const int64_t i = (int64_t)(v * D + k);
+            w[v * D + k] = (float)((i * 7 % 101) - 50) / 13.0f;
+            cols[v * D + k] = (uint32_t)k;
+        }
+    }
+    off[D] = D * D;
+    // packed rows: code 00 -> 0, 10 -> 1, 11 -> 2
+    std::vector<uint8_t> rows(D * (D / 4), 0);
+    for (uint64_t v = 0; v < D; ++v)
+        for (uint64_t p = 0; p < D; ++p) {
+            const uint32_t d = (uint32_t)((v * D + p) % 3), code = d == 0 ? 0u : (d == 1 ? 2u : 3u);
+            rows[v * (D / 4) + p / 4] |= (uint8_t)(code << (2 * (p % 4)));
+        }
+    gnm_problem pr{};
+    pr.num_people = pr.total_people_in_fam = D;
+    pr.bytes_per_variant = D / 4;
+    pr.num_scores = pr.num_variants = D;
+    pr.nnz = D * D;
+    pr.sparse_weights = w.data();
+    pr.sparse_missing_corrections = co.data();
+    pr.sparse_score_columns = cols.data();
+    pr.sparse_row_offsets = off.data();
+    pr.precision = GNM_PRECISION_I8X4;
+    pr.device = device;
+    gnm_ctx *c = new gnm_ctx();
+    uint8_t *d_rows = nullptr;
+    bool ok = build_context(&pr, c) == GNM_OK && hipMalloc(&d_rows, rows.size()) == hipSuccess &&
+              hipMemcpy(d_rows, rows.data(), rows.size(), hipMemcpyHostToDevice) == hipSuccess &&
+              score_device_rows(c, d_rows, D / 4, 0, D) == GNM_OK && hipStreamSynchronize(c->stream) == hipSuccess;
+    std::vector<double> S(D * D);
+    std::vector<uint32_t> C(D * D);
+    ok = ok && hipMemcpy(S.data(), c->d_scores, S.size() * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
+         hipMemcpy(C.data(), c->d_counts, C.size() * sizeof(uint32_t), hipMemcpyDeviceToHost) == hipSuccess;
+    if (d_rows) (void)hipFree(d_rows);
+    destroy_context(c);
+    if (!ok) return false;
+    const double qmax = limb_qmax(4, true);
+    for (uint64_t k = 0; k < D; ++k) {
+        double wmax = 0.0;
+        for (uint64_t v = 0; v < D; ++v) wmax = std::max(wmax, std::fabs((double)w[v * D + k]));
+        for (uint64_t p = 0; p < D; ++p) {
+            double exact = 0.0, dsum = 0.0;
+            for (uint64_t v = 0; v < D; ++v) {
+                const double d = (double)((v * D + p) % 3);
+                exact += d * (double)w[v * D + k];
+                dsum += d;
+            }
+            if (C[p * D + k] != 0 || !(std::fabs(S[p * D + k] - exact) <= dsum * wmax / qmax * (1.0 + 1e-9)))
+                return false;
+        }
+    }
+    return true;
+}
+
+std::mutex g_auto_mu;
+std::map<int, int> g_auto;   // device -> precision AUTO resolves to
+
+// AUTO: split-int8 (I8X4) once its known-answer check has passed on the
+// device, else FP32, decided once per device and process (the reference
+// keeps exact FP32 when its self-test fails, score/cuda_backend.rs:1183-1207).
+int resolve_auto(int device) {
+    std::lock_guard<std::mutex> lk(g_auto_mu);
+    auto it = g_auto.find(device);
+    if (it != g_auto.end()) return it->second;
+    int prec = GNM_PRECISION_FP32;
+    try {
+        if (hipSetDevice(device) == hipSuccess && split_self_test(device)) prec = GNM_PRECISION_I8X4;
+    } catch (...) {
+    }
+    g_auto.emplace(device, prec);
+    return prec;
+}
+
 }  // namespace
 
 extern "C" {
@@ -663,8 +757,10 @@ static gnm_status create_impl(const gnm_problem *problem, gnm_ctx **out_ctx, boo
         if (gate && !gnm_size_gate(problem->num_people, problem->num_scores, problem->num_variants))
             return fail(GNM_DECLINED, "problem size below the device threshold");
         if (gnm_device_count() == 0) return fail(GNM_ERR_NO_DEVICE, "no visible gfx950 device");
+        gnm_problem resolved = *problem;
+        if (resolved.precision == GNM_PRECISION_AUTO) resolved.precision = resolve_auto(resolved.device < 0 ? 0 : resolved.device);
         gnm_ctx *c = new gnm_ctx();
-        st = build_context(problem, c);
+        st = build_context(&resolved, c);
         if (st != GNM_OK) {
             std::string msg = g_last_error;
             destroy_context(c);
@@ -1115,9 +1211,13 @@ gnm_status gnm_dense_forward_device(const uint8_t *d_rows, uint64_t bytes_per_va
         if (V && (!d_rows || !d_weights)) return fail(GNM_ERR_INVALID_ARG, "NULL device pointer");
         if (!d_scores || !d_missing_per_person) return fail(GNM_ERR_INVALID_ARG, "NULL output pointer");
         if (bytes_per_variant < (N + 3) / 4) return fail(GNM_ERR_INVALID_ARG, "bytes_per_variant too small");
-        if (precision < GNM_PRECISION_FP32 || precision > GNM_PRECISION_I8X4)
+        if (precision < GNM_PRECISION_FP32 || precision > GNM_PRECISION_BF16X4)
             return fail(GNM_ERR_INVALID_ARG, "unknown precision");
-        if (precision == GNM_PRECISION_AUTO) precision = GNM_PRECISION_BF16X3;
+        if (precision == GNM_PRECISION_AUTO) {
+            int dev = 0;
+            GNM_HIP(hipGetDevice(&dev));
+            precision = resolve_auto(dev);
+        }
         hipStream_t st = (hipStream_t)stream_;
         DenseWorkspace *ws = workspace_for_current_device();
         if (!ws) return fail(GNM_ERR_NO_DEVICE, "no current HIP device");
@@ -1212,7 +1312,16 @@ gnm_status gnm_dense_adjoint_device(const uint8_t *d_rows, uint64_t bytes_per_va
         if (V && (!d_rows || !d_grad_weights)) return fail(GNM_ERR_INVALID_ARG, "NULL device pointer");
         if (!d_grad_scores) return fail(GNM_ERR_INVALID_ARG, "NULL grad_scores");
         if (bytes_per_variant < (N + 3) / 4) return fail(GNM_ERR_INVALID_ARG, "bytes_per_variant too small");
-        if (precision == GNM_PRECISION_AUTO) precision = GNM_PRECISION_BF16X3;
+        if (precision < GNM_PRECISION_FP32 || precision > GNM_PRECISION_BF16X4)
+            return fail(GNM_ERR_INVALID_ARG, "unknown precision");
+        const bool aligned = ((reinterpret_cast<uintptr_t>(d_rows) | bytes_per_variant) & 3u) == 0 &&
+                             bytes_per_variant < (1ull << 25);
+        if (precision == GNM_PRECISION_AUTO) {
+            int dev = 0;
+            GNM_HIP(hipGetDevice(&dev));
+            precision = (aligned && resolve_auto(dev) == GNM_PRECISION_I8X4) ? GNM_PRECISION_I8X4
+                                                                             : GNM_PRECISION_BF16X3;
+        }
         if (is_int8(precision)) {
             // split-int8 adjoint: R limbs per block of kI8FlushRows people
             const uint32_t L = limbs_of(precision);
@@ -1220,7 +1329,7 @@ gnm_status gnm_dense_adjoint_device(const uint8_t *d_rows, uint64_t bytes_per_va
             const uint32_t used = (uint32_t)(L * (Kp / 16));
             const uint32_t tiles = (used + 3) / 4 * 4;
             if (tiles > 16) return fail(GNM_ERR_UNSUPPORTED, "the int8 adjoint supports up to 64 scores per call");
-            if (((reinterpret_cast<uintptr_t>(d_rows) | bytes_per_variant) & 3u) || bytes_per_variant >= (1ull << 25))
+            if (!aligned)
                 return fail(GNM_ERR_UNSUPPORTED,
                             "the int8 adjoint needs 4-byte aligned rows with a stride that is a multiple of 4 "
                             "(below 2^25)");
@@ -1246,13 +1355,15 @@ gnm_status gnm_dense_adjoint_device(const uint8_t *d_rows, uint64_t bytes_per_va
                                       d_grad_weights, st));
             return GNM_OK;
         }
-        if (precision != GNM_PRECISION_BF16X2 && precision != GNM_PRECISION_BF16X3)
-            return fail(GNM_ERR_UNSUPPORTED, "the adjoint runs in split-BF16 (bf16x2, bf16x3) or split-int8 "
-                                             "(i8x3, i8x4)");
-        const uint32_t L = precision == GNM_PRECISION_BF16X2 ? 2 : 3;
+        if (precision != GNM_PRECISION_BF16X2 && precision != GNM_PRECISION_BF16X3 && precision != GNM_PRECISION_BF16X4)
+            return fail(GNM_ERR_UNSUPPORTED, "the adjoint runs in split-BF16 (bf16x2, bf16x3, bf16x4) or "
+                                             "split-int8 (i8x3, i8x4)");
+        const uint32_t L = limbs_of(precision);
         const uint64_t Kp = (K + 15) / 16 * 16;
         const uint32_t tiles = (uint32_t)(L * (Kp / 16));
-        if (tiles > 12) return fail(GNM_ERR_UNSUPPORTED, "adjoint supports up to 64 scores (bf16x3) per call");
+        if (tiles > 12)
+            return fail(GNM_ERR_UNSUPPORTED, "the split-BF16 adjoint supports up to 12 limb tiles per call "
+                                             "(64 scores in bf16x3, 48 in bf16x4)");
         hipStream_t st = (hipStream_t)stream_;
         DenseWorkspace *ws = workspace_for_current_device();
         if (!ws) return fail(GNM_ERR_NO_DEVICE, "no current HIP device");

@@ -28,7 +28,7 @@ __device__ __forceinline__ uint32_t d_flip_word(uint32_t b) {
 
 // integer-valued BF16 limb j (of L) of w at power-of-two unit 2^e
 __device__ __forceinline__ uint16_t limb_bf16(float w, int e, uint32_t L, uint32_t j) {
-    if (w == 0.0f) return 0;
+    if (w == 0.0f || e == kBadExp) return 0;
     int64_t Q = (int64_t)rint(ldexp((double)w, -e));
     int32_t dig = 0;
     for (uint32_t i = L - 1;; --i) {
@@ -54,25 +54,19 @@ __global__ void k_block_exponent(const float *__restrict__ src, uint64_t rows, u
     const uint64_t blk = blockIdx.y;
     const uint64_t r0 = blk * block;
     const uint64_t r1 = r0 + block < rows ? r0 + block : rows;
-    float m = 0.0f;
-    for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) m = fmaxf(m, fabsf(src[r * ld + k]));
-    __shared__ float red[256];
+    // a NaN or infinity anywhere in the block marks it (kBadExp)
+    double m = 0.0;
+    for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) m = nan_max(m, fabs((double)src[r * ld + k]));
+    __shared__ double red[256];
     red[threadIdx.x] = m;
     __syncthreads();
     for (int sft = 128; sft > 0; sft >>= 1) {
-        if ((int)threadIdx.x < sft) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + sft]);
+        if ((int)threadIdx.x < sft) red[threadIdx.x] = nan_max(red[threadIdx.x], red[threadIdx.x + sft]);
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         const double qmax = int8 ? 127.0 * ldexp(1.0, 8 * (L - 1)) : ldexp(1.0, 8 * L) - ldexp(1.0, 8 * (L - 1));
-        const double mm = red[0];
-        int e = 0;
-        if (mm > 0.0) {
-            e = (int)ceil(log2(mm / qmax));
-            while (ldexp(qmax, e) < mm) ++e;
-            while (ldexp(qmax, e - 1) >= mm) --e;
-        }
-        out[blk * cols + k] = e;
+        out[blk * cols + k] = block_exponent_of(red[0], qmax);
     }
 }
 
@@ -129,7 +123,7 @@ __global__ void k_forward_frag(const float *__restrict__ W, uint64_t V, uint64_t
         if (c < ncol) {
             const uint32_t limb = (uint32_t)(c / K);
             const uint64_t k = c % K;
-            scales[blk * ncol + c] = ldexp(1.0, exps[blk * K + k] + 8 * (int)(L - 1 - limb));
+            scales[blk * ncol + c] = limb_scale(exps[blk * K + k], 8 * (int)(L - 1 - limb));
         }
     }
 }
@@ -234,7 +228,7 @@ __global__ void k_adjoint_frag(const float *__restrict__ R, uint64_t N, uint64_t
         dst[j] = val;
     }
     if (g == 0 && s8 == 0 && (chunk * 256) % kFlushRows == 0 && limb < L && k < K)
-        scales[blk * (L * Kp) + c] = ldexp(1.0, exps[blk * K + k] + 8 * (int)(L - 1 - limb));
+        scales[blk * (L * Kp) + c] = limb_scale(exps[blk * K + k], 8 * (int)(L - 1 - limb));
 }
 
 hipError_t launch_adjoint_frag(const float *R, uint64_t N, uint64_t K, uint64_t Kp, uint32_t L, const int *exps,

@@ -625,7 +625,7 @@ hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t
 
 // signed base-256 digit j (of L) of the integer nearest w / 2^e, all digits in [-128, 127]
 __device__ __forceinline__ int32_t limb_i8(float w, int e, uint32_t L, uint32_t j) {
-    if (w == 0.0f) return 0;
+    if (w == 0.0f || e == kBadExp) return 0;
     int64_t Q = (int64_t)rint(ldexp((double)w, -e));
     for (uint32_t i = L - 1; i > j; --i) {
         const int64_t d = ((Q + 128) & 255) - 128;
@@ -676,7 +676,7 @@ __global__ void k_forward_frag_i8(const float *__restrict__ W, uint64_t V, uint6
         if (c < ncol) {
             const uint32_t limb = (uint32_t)(c / K);
             const uint64_t k = c % K;
-            scales[blk * ncol + c] = ldexp(1.0, exps[blk * K + k] + 8 * (int)(L - 1 - limb));
+            scales[blk * ncol + c] = limb_scale(exps[blk * K + k], 8 * (int)(L - 1 - limb));
         }
     }
 }
@@ -739,7 +739,7 @@ __global__ void k_adjoint_frag_i8(const float *__restrict__ R, uint64_t N, uint6
     for (int q = 0; q < 4; ++q) dst[q] = out[q];
     if (col_ok && g == 0 && (ks * kI8Stage) % kI8FlushRows == 0) {
         const uint64_t blk = ks * kI8Stage / kI8FlushRows;
-        scales[blk * (L * Kp) + c] = ldexp(1.0, exps[blk * K + k] + 8 * (int)(L - 1 - limb));
+        scales[blk * (L * Kp) + c] = limb_scale(exps[blk * K + k], 8 * (int)(L - 1 - limb));
     }
 }
 

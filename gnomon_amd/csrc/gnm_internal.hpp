@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+
 namespace gnm {
 
 // Program rows are grouped in k-steps of 32 (the reduction depth of one
@@ -35,6 +37,30 @@ __host__ __device__ inline void limb_digits(int64_t Q, uint32_t L, int32_t *out)
         Q = (Q - d) / 256;
     }
     out[0] = (int32_t)Q;
+}
+
+// Exponent marker of a (scale block, column) holding a non-finite value:
+// its digits are zero and its limb scales NaN, so every score of that column
+// comes out NaN (a NaN or infinite weight times any dosage, 0 included, is
+// NaN in IEEE arithmetic).
+constexpr int kBadExp = 0x40000000;
+
+// max that keeps a NaN once seen (fmax drops it)
+__host__ __device__ inline double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
+
+// smallest e with qmax * 2^e >= m (0 for m == 0); kBadExp for m inf or NaN
+__host__ __device__ inline int block_exponent_of(double m, double qmax) {
+    if (!(m <= 1.7976931348623157e308)) return kBadExp;
+    if (m == 0.0) return 0;
+    int e = (int)ceil(log2(m / qmax));
+    while (ldexp(qmax, e) < m) ++e;
+    while (ldexp(qmax, e - 1) >= m) --e;
+    return e;
+}
+
+// FP64 scale of limb digits at unit 2^(e + shift)
+__host__ __device__ inline double limb_scale(int e, int shift) {
+    return e == kBadExp ? __builtin_nan("") : ldexp(1.0, e + shift);
 }
 
 // |q| <= 256 is exact in BF16: the top half of its f32 pattern

@@ -56,19 +56,14 @@ __global__ __launch_bounds__(256) void k_plan_exps(const double *__restrict__ wr
     const uint64_t r0 = blk * fr, r1 = r0 + fr < R ? r0 + fr : R;
     double m = 0.0;
     if (k < K)
-        for (uint64_t r = r0 + q; r < r1; r += 4) m = fmax(m, fabs(wrow[r * K + k]));
+        for (uint64_t r = r0 + q; r < r1; r += 4) m = nan_max(m, fabs(wrow[r * K + k]));
     red[q][kk] = m;
     __syncthreads();
     if (q != 0 || k >= K) return;
-    m = fmax(fmax(red[0][kk], red[1][kk]), fmax(red[2][kk], red[3][kk]));
-    int e = 0;
-    if (m > 0.0) {   // smallest e with qmax * 2^e >= m
-        e = (int)ceil(log2(m / qmax));
-        while (ldexp(qmax, e) < m) ++e;
-        while (ldexp(qmax, e - 1) >= m) --e;
-    }
+    m = nan_max(nan_max(red[0][kk], red[1][kk]), nan_max(red[2][kk], red[3][kk]));
+    const int e = block_exponent_of(m, qmax);
     uexp[blk * K + k] = e;
-    for (uint32_t j = 0; j < L; ++j) scales[blk * ncol + j * K + k] = ldexp(1.0, e + 8 * (int)(L - 1 - j));
+    for (uint32_t j = 0; j < L; ++j) scales[blk * ncol + j * K + k] = limb_scale(e, 8 * (int)(L - 1 - j));
 }
 
 // one thread per (k-step, tile, lane): 16 bytes, rows 64ks + 16g + b of
@@ -95,10 +90,11 @@ __global__ __launch_bounds__(256) void k_plan_frag_i8(const double *__restrict__
                 const uint32_t limb = col / (uint32_t)K;
                 const uint64_t k = col % K;
                 const double w = wrow[r * K + k];
-                if (w != 0.0) {
-                    const uint64_t blk = ks * kI8KStep / fr;
+                const uint64_t blk = ks * kI8KStep / fr;
+                const int e = uexp[blk * K + k];
+                if (w != 0.0 && e != kBadExp) {
                     int32_t dig[4];
-                    limb_digits((int64_t)rint(ldexp(w, -uexp[blk * K + k])), L, dig);
+                    limb_digits((int64_t)rint(ldexp(w, -e)), L, dig);
                     val = (uint8_t)(int8_t)dig[limb];
                 }
             }
@@ -135,10 +131,11 @@ __global__ __launch_bounds__(256) void k_plan_frag_bf16(const double *__restrict
                 const uint32_t limb = col / (uint32_t)K;
                 const uint64_t k = col % K;
                 const double w = wrow[r * K + k];
-                if (w != 0.0) {
-                    const uint64_t blk = ks * kKStep / kFlushRows;
+                const uint64_t blk = ks * kKStep / kFlushRows;
+                const int e = uexp[blk * K + k];
+                if (w != 0.0 && e != kBadExp) {
                     int32_t dig[4];
-                    limb_digits((int64_t)rint(ldexp(w, -uexp[blk * K + k])), L, dig);
+                    limb_digits((int64_t)rint(ldexp(w, -e)), L, dig);
                     val = bf16_of_int(dig[limb]);
                 }
             }

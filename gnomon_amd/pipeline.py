@@ -32,7 +32,7 @@ class ScoringContext:
     """Owns one ``gnm_ctx``.  Arrays passed in are kept alive for its lifetime."""
 
     def __init__(self, csr: ScoreCsr, num_people: int, total_people_in_fam: int | None = None,
-                 out_to_fam=None, precision: str = "bf16x3", device: int = 0, force: bool = False):
+                 out_to_fam=None, precision: str = "auto", device: int = 0, force: bool = False):
         L = lib()
         self.csr = csr
         self.precision = precision
@@ -138,7 +138,7 @@ class ScoringContext:
 
 
 def try_run_hip(csr: ScoreCsr, rows: np.ndarray, num_people: int, total_people_in_fam: int | None = None,
-                out_to_fam=None, precision: str = "bf16x3", device: int = 0):
+                out_to_fam=None, precision: str = "auto", device: int = 0):
     """Drop-in for try_run_cuda over an in-memory .bed image (rows [V, bpv])."""
     L = lib()
     if not L.gnm_size_gate(num_people, csr.num_scores, csr.num_variants):

@@ -48,26 +48,38 @@ typedef enum gnm_status {
  *           genotype (no split, no matrix cores).  Replaces the reference's
  *           FP32 kernels (score/kernel.rs:124-176, cuda_backend.rs:2554-2584),
  *           whose FP32 partial sums it makes exact.
- *   BF16X2/BF16X3: split-BF16 MFMA.  Each weight column is written as 2 or 3
- *           integer-valued BF16 limbs times a power-of-two scale per
- *           (variant block, column); dosages 0/1/2 are exact in BF16, so every
- *           product and every FP32 partial sum inside a variant block is exact;
- *           blocks are combined in FP64.  Replaces the split-FP16 tensor-core
- *           path (cuda_backend.rs:2360-2415).
+ *   BF16X2/BF16X3/BF16X4: split-BF16 MFMA.  Each weight column is written
+ *           as 2, 3 or 4 integer-valued BF16 limbs (signed base-256 digits)
+ *           times a power-of-two scale per (variant block, column); dosages
+ *           0/1/2 are exact in BF16, so every product and every FP32 partial
+ *           sum inside a variant block is exact; blocks are combined in FP64.
+ *           BF16X4 represents 32 bits of each block maximum, so every FP32
+ *           weight within 2^8 of its block maximum is represented exactly
+ *           and the rest to 2^-32 of that maximum; BF16X3 24 bits, BF16X2 16.
+ *           Replaces the split-FP16 tensor-core path
+ *           (cuda_backend.rs:2360-2415).
  *   I8X3/I8X4: split-int8 MFMA (v_mfma_i32_16x16x64_i8).  Each weight column
  *           is written as 3 or 4 signed 8-bit integer limbs times a power-of-two
  *           scale per (variant block, column); products and int32 block sums
  *           are exact; blocks are combined in FP64.  I8X4 represents 31 bits
  *           of each block maximum, I8X3 23 bits.  Same role as the split paths
  *           above (no reference counterpart for the int8 limbs themselves).
- *   AUTO:   BF16X3. */
+ *   AUTO:   I8X4 once a known-answer check of that path has passed on the
+ *           device (the calibrate_fp16 inputs of cuda_backend.rs:2424-2494,
+ *           checked against exact host sums; run once per device and
+ *           process), else FP32 -- the reference likewise enables its split
+ *           path only after its self-test (cuda_backend.rs:1176-1196).
+ *   Weights and corrections must be finite (GNM_ERR_INVALID_ARG otherwise),
+ *   as the reference refuses its split path for non-finite weights
+ *   (cuda_backend.rs:1178-1181). */
 typedef enum gnm_precision {
     GNM_PRECISION_FP32 = 0,
     GNM_PRECISION_BF16X2 = 1,
     GNM_PRECISION_BF16X3 = 2,
     GNM_PRECISION_AUTO = 3,
     GNM_PRECISION_I8X3 = 4,
-    GNM_PRECISION_I8X4 = 5
+    GNM_PRECISION_I8X4 = 5,
+    GNM_PRECISION_BF16X4 = 6
 } gnm_precision;
 
 /* The slice of PreparationResult (score/types.rs:166-365) the hot path reads. */
@@ -194,8 +206,9 @@ gnm_status gnm_unpack_device(const uint8_t *d_rows, uint64_t bytes_per_variant,
 /* Dense effect-allele forward, S[p,k] = sum_v g_pv W[v,k] with
  * g = d (flip 0) or 2-d (flip 1), missing -> 0; missing_per_person[p] =
  * number of missing calls.  W is [num_variants][num_scores] f32, S is
- * [num_people][num_scores] f64 (overwritten).  New (no reference counterpart):
- * the forward of the differentiable path. */
+ * [num_people][num_scores] f64 (overwritten).  A NaN or infinity in W
+ * makes the affected columns of S NaN.  New (no reference counterpart): the
+ * forward of the differentiable path. */
 gnm_status gnm_dense_forward_device(const uint8_t *d_rows, uint64_t bytes_per_variant,
                                     uint64_t num_variants, uint64_t num_people,
                                     const uint8_t *d_flip, const float *d_weights,
@@ -204,10 +217,13 @@ gnm_status gnm_dense_forward_device(const uint8_t *d_rows, uint64_t bytes_per_va
 
 /* Adjoint dL/dW = G^T dL/dS, recomputing G from the .bed (no stored
  * dosages).  grad_scores [num_people][num_scores] f32, grad_weights
- * [num_variants][num_scores] f32 (overwritten).  precision: BF16X2/BF16X3
- * (AUTO = BF16X3) split dL/dS into BF16 limbs per block of 8192 people;
- * I8X3/I8X4 into int8 limbs per block of 65536 people (rows 4-byte aligned,
- * bytes_per_variant a multiple of 4).  Up to 64 scores per call.  New (no
+ * [num_variants][num_scores] f32 (overwritten).  precision: BF16X2/BF16X3/
+ * BF16X4 split dL/dS into BF16 limbs per block of 8192 people (up to 12 limb
+ * tiles: 64 scores in BF16X3, 48 in BF16X4); I8X3/I8X4 into int8 limbs per
+ * block of 65536 people (rows 4-byte aligned, bytes_per_variant a multiple
+ * of 4 below 2^25; up to 64 scores); AUTO = I8X4 where the rows allow it and
+ * the split self-test passed, else BF16X3.  A NaN or infinity in
+ * grad_scores makes the affected columns of grad_weights NaN.  New (no
  * reference counterpart). */
 gnm_status gnm_dense_adjoint_device(const uint8_t *d_rows, uint64_t bytes_per_variant,
                                     uint64_t num_variants, uint64_t num_people,

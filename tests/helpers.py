@@ -38,3 +38,38 @@ def gpu_tests_genotype(person, variant):
     """score/tests/gpu_tests.rs:611-619."""
     key = (person * 131 + variant * 17 + 23) % 31
     return np.select([key <= 1, key <= 9, key <= 20], [-1, 0, 1], default=2)
+
+
+def per_sample_rel(S, S64, floor=1e-3):
+    """Per score: max over people with |S64| >= floor * max|S64| of
+    |S - S64| / |S64| (0 for a score whose oracle column is all zero)."""
+    S = np.asarray(S, np.float64)
+    S64 = np.asarray(S64, np.float64)
+    out = np.zeros(S64.shape[1])
+    for k in range(S64.shape[1]):
+        ref = S64[:, k]
+        m = np.abs(ref).max()
+        if m == 0:
+            continue
+        sel = np.abs(ref) >= floor * m
+        out[k] = (np.abs(S[sel, k] - ref[sel]) / np.abs(ref[sel])).max()
+    return out
+
+
+def oracle_fp64_parallel(orc, rows, csr, num_people, threads=0):
+    """The FP64 oracle over slices of people in a thread pool (the oracle's C
+    entry point runs without the GIL); identical to one orc.score_fp64 call."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    threads = threads or min(16, os.cpu_count() or 1)
+    edges = np.linspace(0, num_people, threads + 1).astype(np.int64)
+    parts = [(int(a), int(b)) for a, b in zip(edges[:-1], edges[1:]) if b > a]
+
+    def run(ab):
+        a, b = ab
+        return orc.score_fp64(rows, csr, b - a, np.arange(a, b, dtype=np.uint32))
+
+    with ThreadPoolExecutor(len(parts)) as ex:
+        res = list(ex.map(run, parts))
+    return np.concatenate([r[0] for r in res]), np.concatenate([r[1] for r in res])

@@ -105,3 +105,30 @@ def test_oracle_is_not_imported_by_the_product():
             if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")):
                 text = open(os.path.join(dirpath, f), errors="replace").read()
                 assert "oracle" not in re.sub(r"#.*|//.*", "", text).replace("the oracle", ""), f
+
+
+@pytest.mark.parametrize("field,value", [("weights", np.nan), ("weights", np.inf), ("corrections", -np.inf)])
+def test_non_finite_weights_rejected_without_device(field, value):
+    """Non-finite CSR weights or corrections are refused before any device
+    work, as the reference refuses its split path for them
+    (score/cuda_backend.rs:1178-1181)."""
+    L = _lib.lib()
+    csr = csr_from_dense(np.ones((4, 2), np.float32), np.array([False, True, False, True]))
+    getattr(csr, field)[3] = value
+    ctx = ctypes.c_void_p()
+    for prec in (0, 3, 5, 6):
+        pr = _problem(csr, 10, precision=prec)
+        assert L.gnm_create_forced(ctypes.byref(pr), ctypes.byref(ctx)) == _lib.GNM_ERR_INVALID_ARG
+        assert "not finite" in _lib.last_error()
+    assert not ctx.value
+
+
+def test_precision_codes():
+    """bf16x4 is code 6; codes past it are refused."""
+    assert _lib.PRECISION["bf16x4"] == 6
+    L = _lib.lib()
+    csr = csr_from_dense(np.ones((4, 2), np.float32), np.zeros(4, bool))
+    ctx = ctypes.c_void_p()
+    pr = _problem(csr, 10, precision=7)
+    assert L.gnm_create_forced(ctypes.byref(pr), ctypes.byref(ctx)) == _lib.GNM_ERR_INVALID_ARG
+    assert "unknown precision" in _lib.last_error()

@@ -25,7 +25,7 @@ pytestmark = pytest.mark.gpu
 
 BLOCK = 8192   # rows (forward: variants; adjoint: people) per scale block
 I8_BLOCK = 65536   # variants per scale block of the split-int8 forward
-LIMBS = {"bf16x3": 3, "bf16x2": 2, "i8x4": 4, "i8x3": 3}
+LIMBS = {"bf16x4": 4, "bf16x3": 3, "bf16x2": 2, "i8x4": 4, "i8x3": 3}
 
 
 def _qmax(precision):
@@ -71,7 +71,7 @@ def _dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16x2", "i8x4", "i8x3"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x4", "bf16x3", "bf16x2", "i8x4", "i8x3"])
 def test_dense_forward_matches_fp64(orc, precision):
     N, V, K = 3001, 9000, 16          # two variant scale blocks, ragged N and V
     _flush(f"dense forward {precision} N={N} V={V} K={K}")
@@ -89,7 +89,7 @@ def test_dense_forward_matches_fp64(orc, precision):
         bound = _limb_bound(G, W.astype(np.float64), precision,
                             I8_BLOCK if precision.startswith("i8") else BLOCK)
         assert np.all(err <= bound + 1e-12 * np.abs(S64).max()), (err - bound).max()
-        if precision in ("bf16x3", "i8x4"):
+        if precision in ("bf16x4", "bf16x3", "i8x4"):
             assert err.max() / np.abs(S64).max() <= 1e-6
 
 
@@ -249,3 +249,64 @@ def test_dense_forward_and_adjoint_tiny_shapes(orc, precision, N, V, K):
     Gt = _effect_dosage(dos, flip).T
     bound = _limb_bound(Gt, R.astype(np.float64), precision, I8_BLOCK if precision.startswith("i8") else BLOCK)
     assert np.all(np.abs(dW - dW64) <= bound + 2.0 ** -23 * np.abs(dW64) + 1e-30)
+
+
+def test_operands_checked_before_launch():
+    """Wrong device, shape or dtype of W, R or flip raise ValueError before
+    any kernel runs."""
+    _flush("operand checks")
+    N, V, K = 64, 32, 4
+    _, rows, flip, W = _data(N, V, K, 3)
+    bed, fl = _dev(rows), _dev(flip)
+    with pytest.raises(ValueError):
+        ag.dense_forward(torch.from_numpy(W), bed, fl, N)                 # W on the host
+    with pytest.raises(ValueError):
+        ag.dense_forward(_dev(W), bed, fl.to(torch.int64), N)             # wide flip
+    with pytest.raises(ValueError):
+        ag.dense_forward(_dev(W), bed, fl[:-1].contiguous(), N)           # short flip
+    with pytest.raises(ValueError):
+        ag.dense_forward(_dev(W), bed, fl, N, "bf16x9")
+    R = torch.zeros(N, K, device="cuda")
+    with pytest.raises(ValueError):
+        ag.dense_adjoint(R.cpu(), bed, fl, V)
+    with pytest.raises(ValueError):
+        ag.dense_adjoint(R[:-1], bed, fl, V)
+    with pytest.raises(ValueError):
+        ag.dense_adjoint(R, bed, fl.cpu(), V)
+    torch.cuda.synchronize()
+    S, _ = ag.dense_forward(_dev(W), bed, fl, N, "i8x4")                # still usable after
+    assert torch.isfinite(S).all()
+
+
+@pytest.mark.parametrize("precision", ["bf16x4", "i8x4", "fp32"])
+def test_non_finite_weight_gives_nan_column(precision):
+    """A NaN or infinite weight makes its score column NaN for every person
+    (NaN times any dosage, 0 included); other columns stay finite and equal
+    to the run without it."""
+    N, V, K = 300, 700, 3
+    _flush(f"non-finite weight {precision}")
+    _, rows, flip, W = _data(N, V, K, 5)
+    S0, _ = ag.dense_forward(_dev(W), _dev(rows), _dev(flip), N, precision)
+    for bad in (np.nan, np.inf):
+        Wb = W.copy()
+        Wb[123, 1] = bad
+        S, _ = ag.dense_forward(_dev(Wb), _dev(rows), _dev(flip), N, precision)
+        torch.cuda.synchronize()
+        S, s0 = S.cpu().numpy(), S0.cpu().numpy()
+        assert np.isnan(S[:, 1]).all()
+        assert np.array_equal(S[:, [0, 2]], s0[:, [0, 2]])
+
+
+@pytest.mark.parametrize("precision", ["i8x4", "bf16x3"])
+def test_nan_gradient_reaches_dw(precision):
+    """A NaN in dL/dS makes the matching column of dL/dW NaN."""
+    N, V, K = 300, 700, 3
+    _flush(f"NaN gradient {precision}")
+    _, rows, flip, _ = _data(N, V, K, 6)
+    R = torch.randn(N, K, device="cuda")
+    R[17, 2] = float("nan")
+    dW = ag.dense_adjoint(R, _dev(rows), _dev(flip), V, precision)
+    torch.cuda.synchronize()
+    dW = dW.cpu().numpy()
+    assert np.isnan(dW[:, 2]).all()
+    assert np.isfinite(dW[:, :2]).all()
