@@ -36,6 +36,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # arithmetic of each path: limb type -> exact block accumulator -> cross-block reduction
 DTYPE = {"fp32": "f32*{0,1,2}->f64", "bf16x2": "bf16x2->f32->f64", "bf16x3": "bf16x3->f32->f64",
+         "bf16x4": "bf16x4->f32->f64",
          "i8x3": "i8x3->i32->f64", "i8x4": "i8x4->i32->f64"}
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
@@ -466,7 +467,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="plain_fp32_biobank")
-    ap.add_argument("--precision", default="i8x4", choices=["fp32", "bf16x2", "bf16x3", "i8x3", "i8x4"])
+    ap.add_argument("--precision", default="i8x4", choices=["fp32", "bf16x2", "bf16x3", "bf16x4", "i8x3", "i8x4"])
     ap.add_argument("--compare", default="fp32,bf16x3",
                     help="comma-separated further paths timed on the same rows for the report ('' = none)")
     ap.add_argument("--variants", type=int, default=0, help="override the variant count")
