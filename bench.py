@@ -480,7 +480,8 @@ def main():
     ap.add_argument("--host-rows", action="store_true",
                     help="also time a pass with the rows in host memory (PCIe-inclusive, not the headline value)")
     ap.add_argument("--launch", default="", metavar="TP,BS",
-                    help="int8 launch shape (tiles per pass, batch stages) instead of the default")
+                    help="launch shape of the headline path's MFMA kernel (tiles per pass, batch stages) instead "
+                         "of the default")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise copy of MIB MiB (read + write) three times; used to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters")
@@ -527,7 +528,7 @@ def main():
     def run_path(precision):
         t1 = time.time()
         ctx = ScoringContext(csr, N, precision=precision, force=True)
-        if args.launch and precision.startswith("i8"):
+        if args.launch and precision == args.precision:
             ctx.set_launch_config(*(int(x) for x in args.launch.split(",")))
         log(f"[{precision}] plan built ({time.time() - t1:.1f}s)")
         for _ in range(args.warmup):

@@ -12,7 +12,8 @@ from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCE_NAMES = ("gnm_kernels.hip", "gnm_dense.hip", "gnm_i8.hip", "gnm_sparse.hip", "gnm_plan.hip", "gnm_api.hip")
+SOURCE_NAMES = ("gnm_kernels.hip", "gnm_dense.hip", "gnm_i8.hip", "gnm_bf16.hip", "gnm_sparse.hip", "gnm_plan.hip",
+                "gnm_api.hip")
 SOURCES = [os.path.join(HERE, "csrc", f) for f in SOURCE_NAMES]
 OUTPUT = os.path.join(HERE, "libgnomon_hip.so")
 OBJDIR = os.path.join(HERE, "csrc", "_build")

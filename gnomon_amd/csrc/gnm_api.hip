@@ -398,10 +398,17 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
     pl.score_class = c->d_score_class;
     pl.limbs = c->limbs;
     pl.int8 = int8 ? 1u : 0u;
-    if (int8) {
+    {
+        // selector byte of each program row for the MFMA kernels' decode
+        // (bit 7: a count row, when the MFMA kernels count missing calls
+        // themselves: one membership class)
         std::vector<uint8_t> flipbyte(Rpad, 0);
-        for (uint64_t r = 0; r < R; ++r)
-            if ((flipmask[r >> 5] >> (r & 31)) & 1u) flipbyte[r] = 4;
+        const uint8_t sel = int8 ? kFlipSelI8 : kFlipSelBf;
+        const bool vc = c->precision != GNM_PRECISION_FP32 && c->nclass == 1;
+        for (uint64_t r = 0; r < R; ++r) {
+            if ((flipmask[r >> 5] >> (r & 31)) & 1u) flipbyte[r] = sel;
+            if (vc && prog_count_row[r]) flipbyte[r] |= 0x80u;
+        }
         uint8_t *d_flipbyte;
         if ((st = c->upload(&d_flipbyte, flipbyte)) != GNM_OK) return st;
         pl.flipbyte = d_flipbyte;
@@ -413,7 +420,10 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
     const uint32_t L = c->limbs;
     pl.ncol = mfma_weights ? (uint32_t)(L * K) : 0;
     pl.wtiles = (pl.ncol + 15) / 16;
-    pl.ctiles = mfma_counts ? (c->nclass + 15) / 16 : 0;
+    // one membership class: the MFMA kernels count missing calls on the
+    // vector ALUs (no count tile)
+    pl.vcount = (mfma_weights && c->nclass == 1) ? 1u : 0u;
+    pl.ctiles = (mfma_counts && !pl.vcount) ? (c->nclass + 15) / 16 : 0;
     const uint32_t tiles = pl.wtiles + pl.ctiles;
 
     // ---- the weight plan, built on the device from the CSR (gnm_plan.hip):
@@ -451,8 +461,7 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
             pl.k32 = k32;
         }
         if (tiles) {
-            const uint64_t kstep = int8 ? (uint64_t)kI8KStep : (uint64_t)kKStep;
-            const uint64_t nks = Rpad / kstep;
+            const uint64_t nst = Rpad / kI8KStep;   // 64-row stages
             const uint64_t fr = (uint64_t)flush_rows(int8);   // rows per scale block
             const uint64_t nblk = (Rpad + fr - 1) / fr;
             int *d_uexp = nullptr;
@@ -464,10 +473,11 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
                                          c->stream));
                 pl.scales = d_scales;
             }
-            // 16 bytes per (k-step, tile, lane): 16 int8 or 8 BF16 values
+            // 16 bytes per (stage, tile, lane) of 16 int8 values, or per
+            // (stage, tile, k-step, lane) of 8 BF16 values
             uint4 *d_frag;
-            if ((st = c->alloc(&d_frag, nks * tiles * 64)) != GNM_OK) return st;
-            GNM_HIP(launch_plan_frag(int8, d_wrow, R, K, d_uexp, fr, L, pl.ncol, pl.wtiles, tiles, nks, d_count_row,
+            if ((st = c->alloc(&d_frag, nst * tiles * (int8 ? 64 : 128))) != GNM_OK) return st;
+            GNM_HIP(launch_plan_frag(int8, d_wrow, R, K, d_uexp, fr, L, pl.ncol, pl.wtiles, tiles, nst, d_count_row,
                                      d_rowcls, cw, c->nclass, d_frag, c->stream));
             pl.bfrag = (const uint32_t *)d_frag;
         }
@@ -480,9 +490,9 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
     if ((st = c->alloc(&c->d_counts, cells)) != GNM_OK) return st;
     GNM_HIP(launch_fill_f64(c->d_scores, cells, 0.0, c->stream));
     GNM_HIP(launch_fill_u32(c->d_counts, cells, 0u, c->stream));
-    c->splits = mfma_weights ? (int8 ? i8_splits(c->N, tiles) : mfma_splits(c->N, tiles)) : fp32_splits(c->N, K);
+    c->splits = mfma_weights ? (int8 ? i8_splits(c->N, tiles) : bf16_splits(c->N, tiles)) : fp32_splits(c->N, K);
     if (!mfma_weights && !c->fp32_count_in_kernel)
-        c->splits = std::max(c->splits, mfma_splits(c->N, tiles));
+        c->splits = std::max(c->splits, bf16_splits(c->N, tiles));
     const uint64_t ncol_part = mfma_weights ? (uint64_t)L * K : K;
     if ((st = c->alloc(&c->d_part, (uint64_t)c->splits * c->N * ncol_part)) != GNM_OK) return st;
     if ((st = c->alloc(&c->d_cpart, (uint64_t)c->splits * c->N * c->nclass)) != GNM_OK) return st;
@@ -578,15 +588,16 @@ gnm_status score_device_rows(gnm_ctx *c, const uint8_t *d_rows, uint64_t stride,
         GNM_HIP(launch_score_fp32(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, c->K, S,
                                   c->fp32_count_in_kernel, c->d_part, c->d_cpart, c->stream));
         if (!c->fp32_count_in_kernel)
-            GNM_HIP(launch_score_mfma(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, S, 0,
-                                      c->plan.ctiles, identity, c->d_part, c->d_cpart, c->stream));
+            GNM_HIP(launch_score_bf16(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, S, 0, c->plan.ctiles, identity,
+                                      c->d_part, c->d_cpart, c->stream));
     } else if (c->plan.int8) {
         GNM_HIP(launch_score_i8(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, S, 0,
                                 c->plan.wtiles + c->plan.ctiles, identity, c->d_part, c->d_cpart, c->stream,
                                 c->tiles_per_pass, c->batch_stages));
     } else {
-        GNM_HIP(launch_score_mfma(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, S, 0,
-                                  c->plan.wtiles + c->plan.ctiles, identity, c->d_part, c->d_cpart, c->stream));
+        GNM_HIP(launch_score_bf16(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, S, 0,
+                                  c->plan.wtiles + c->plan.ctiles, identity, c->d_part, c->d_cpart, c->stream,
+                                  c->tiles_per_pass));
     }
     GNM_HIP(hipEventRecord(c->ev_main1, c->stream));
     GNM_HIP(launch_reduce(c->d_part, c->d_cpart, S, c->N, c->K,
@@ -1090,6 +1101,17 @@ gnm_status gnm_unpack_device(const uint8_t *d_rows, uint64_t bytes_per_variant, 
 
 gnm_status gnm_set_launch_config(gnm_ctx *ctx, uint32_t tiles_per_pass, uint32_t batch_stages) {
     if (!ctx) return fail(GNM_ERR_INVALID_ARG, "ctx is NULL");
+    if (!is_int8(ctx->precision)) {
+        // split-BF16 kernel: tile groups of 3, 5, 6, 8 or 9, two-stage batches
+        if (tiles_per_pass != 0 && tiles_per_pass != 3 && tiles_per_pass != 5 && tiles_per_pass != 6 &&
+            tiles_per_pass != 8 && tiles_per_pass != 9)
+            return fail(GNM_ERR_INVALID_ARG, "tiles_per_pass must be 0, 3, 5, 6, 8 or 9 on the split-BF16 path");
+        if (batch_stages != 0 && batch_stages != 2)
+            return fail(GNM_ERR_INVALID_ARG, "batch_stages must be 0 or 2 on the split-BF16 path");
+        ctx->tiles_per_pass = tiles_per_pass;
+        ctx->batch_stages = batch_stages;
+        return GNM_OK;
+    }
     if (tiles_per_pass != 0 && tiles_per_pass != 3 && tiles_per_pass != 4 && tiles_per_pass != 5 &&
         tiles_per_pass != 6 && tiles_per_pass != 8 && tiles_per_pass != 18)
         return fail(GNM_ERR_INVALID_ARG, "tiles_per_pass must be 0, 3, 4, 5, 6, 8 or 18");
@@ -1226,9 +1248,9 @@ gnm_status gnm_dense_forward_device(const uint8_t *d_rows, uint64_t bytes_per_va
         const bool int8 = is_int8(precision);
         const uint64_t Rpad = std::max<uint64_t>((V + 63) / 64 * 64, 64);
         const uint32_t Lw = L == 1 ? 0 : L;
-        const uint32_t ntl = (uint32_t)((Lw * K + 15) / 16 + 1);
+        const uint32_t ntl = (uint32_t)((Lw * K + 15) / 16);   // weight tiles; missing calls counted on the VALU
         const uint32_t S = precision == GNM_PRECISION_FP32 ? fp32_splits(N, K)
-                                                           : (int8 ? i8_splits(N, ntl) : mfma_splits(N, ntl));
+                                                           : (int8 ? i8_splits(N, ntl) : bf16_splits(N, ntl));
         gnm_status s;
         DevicePlan pl{};
         uint32_t *prog_var, *flipmask, *prog_cls, *score_class, *cnt_tmp, *cpart;
@@ -1264,33 +1286,34 @@ gnm_status gnm_dense_forward_device(const uint8_t *d_rows, uint64_t bytes_per_va
         } else {
             pl.ncol = (uint32_t)(L * K);
             pl.wtiles = (pl.ncol + 15) / 16;
-            pl.ctiles = 1;
+            pl.ctiles = 0;
+            pl.vcount = 1;   // one class, every row a count row
             const uint32_t tiles = pl.wtiles + pl.ctiles;
-            const uint64_t nks = Rpad / (int8 ? kI8KStep : kKStep);
+            const uint64_t nst = Rpad / kI8KStep;   // 64-row stages
             const uint64_t fr = (uint64_t)flush_rows(int8);
             const uint64_t nblk = (Rpad + fr - 1) / fr;
             int *exps;
-            uint16_t *frag;   // 16 bytes per (k-step, tile, lane) in both layouts
+            uint16_t *frag;   // 16 bytes per (stage, tile, lane) int8, per (stage, tile, k-step, lane) BF16
             double *scales;
+            uint8_t *flipbyte;
             if ((s = ws_typed(ws, 8, nblk * K, st, &exps)) != GNM_OK) return s;
-            if ((s = ws_typed(ws, 9, nks * tiles * 64 * 8, st, &frag)) != GNM_OK) return s;
+            if ((s = ws_typed(ws, 9, nst * tiles * (int8 ? 64 : 128) * 8, st, &frag)) != GNM_OK) return s;
             if ((s = ws_typed(ws, 10, nblk * pl.ncol, st, &scales)) != GNM_OK) return s;
+            if ((s = ws_typed(ws, 14, Rpad, st, &flipbyte)) != GNM_OK) return s;
             GNM_HIP(launch_block_exponent(d_weights, V, K, K, fr, L, int8, exps, st));
+            GNM_HIP(launch_flip_bytes(d_flip, V, Rpad, int8 ? kFlipSelI8 : kFlipSelBf, flipbyte, st));
             pl.scales = scales;
             pl.bfrag = (const uint32_t *)frag;
+            pl.flipbyte = flipbyte;
             if (int8) {
-                uint8_t *flipbyte;
-                if ((s = ws_typed(ws, 14, Rpad, st, &flipbyte)) != GNM_OK) return s;
-                GNM_HIP(launch_flip_bytes(d_flip, V, Rpad, flipbyte, st));
-                GNM_HIP(launch_forward_frag_i8(d_weights, V, K, L, exps, pl.wtiles, tiles, nks, (uint32_t *)frag,
+                GNM_HIP(launch_forward_frag_i8(d_weights, V, K, L, exps, pl.wtiles, tiles, nst, (uint32_t *)frag,
                                                scales, st));
                 pl.int8 = 1;
-                pl.flipbyte = flipbyte;
                 GNM_HIP(launch_score_i8(pl, d_rows, bytes_per_variant, (N + 3) / 4, 0, 0, V, N, S, 0, tiles, true,
                                         part, cpart, st));
             } else {
-                GNM_HIP(launch_forward_frag(d_weights, V, K, L, exps, pl.wtiles, tiles, nks, frag, scales, st));
-                GNM_HIP(launch_score_mfma(pl, d_rows, bytes_per_variant, (N + 3) / 4, 0, 0, V, N, S, 0, tiles,
+                GNM_HIP(launch_forward_frag(d_weights, V, K, L, exps, pl.wtiles, tiles, nst, frag, scales, st));
+                GNM_HIP(launch_score_bf16(pl, d_rows, bytes_per_variant, (N + 3) / 4, 0, 0, V, N, S, 0, tiles,
                                           true, part, cpart, st));
             }
         }

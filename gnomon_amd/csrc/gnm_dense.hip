@@ -1,9 +1,9 @@
 // Device-side plan builders and the adjoint kernel of the differentiable
 // path (dense weights on the device, e.g. a PyTorch-ROCm parameter).
 //
-// Forward:  S[p,k] = sum_v g_pv W[v,k]  -- the scoring kernels of
-//           gnm_kernels.hip with an identity variant program; the split-BF16
-//           B fragments are built here from device weights.
+// Forward:  S[p,k] = sum_v g_pv W[v,k]  -- the scoring kernels with an
+//           identity variant program; the split-BF16 B fragments are built
+//           here from device weights.
 // Adjoint:  dW[v,k] = sum_p g_pv R[p,k] (R = dL/dS), recomputing g from the
 //           packed .bed instead of storing dosages.  MFMA with variants on M,
 //           scores (x limbs) on N and people on the reduction axis: a lane
@@ -80,25 +80,29 @@ hipError_t launch_block_exponent(const float *src, uint64_t rows, uint64_t ld, u
 }
 
 // --------------------------------------------------------------------------
-// forward B fragments from dense W [V][K]: column c = limb*K + k; tile
-// columns beyond ncol and rows beyond V are zero; count tile (class 0) is
-// 1.0 on every real row.  One thread per (k-step, tile, lane).
+// forward B fragments from dense W [V][K] in the split-BF16 scoring
+// kernel's layout [64-row stage][tile][k-step h][lane][8 bf16] (lane (n, g)
+// of k-step h: rows 64 st + 16 g + 8 h + j, column 16 tile + n); column
+// c = limb*K + k; tile columns beyond ncol and rows beyond V are zero; the
+// count tile (class 0) is 1.0 on every real row.  Scale blocks of
+// kBfFlushRows rows.  One thread per (stage, tile, k-step, lane).
 // --------------------------------------------------------------------------
 __global__ void k_forward_frag(const float *__restrict__ W, uint64_t V, uint64_t K, uint32_t L,
                                const int *__restrict__ exps, uint32_t wtiles, uint32_t tiles,
-                               uint64_t nks, uint16_t *__restrict__ frag, double *__restrict__ scales) {
+                               uint64_t nst, uint16_t *__restrict__ frag, double *__restrict__ scales) {
     const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= nks * tiles * 64) return;
+    if (id >= nst * tiles * 128) return;
     const uint32_t lane = id & 63;
-    const uint32_t tile = (uint32_t)((id >> 6) % tiles);
-    const uint64_t ks = (id >> 6) / tiles;
+    const uint32_t h = (uint32_t)((id >> 6) & 1);
+    const uint32_t tile = (uint32_t)((id >> 7) % tiles);
+    const uint64_t stg = (id >> 7) / tiles;
     const uint32_t n = lane & 15, g = lane >> 4;
     const uint64_t ncol = (uint64_t)L * K;
     uint16_t out[8];
-    const uint64_t blk = ks * kKStep / kFlushRows;
+    const uint64_t blk = stg * 64 / kBfFlushRows;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-        const uint64_t r = ks * kKStep + g * 8 + j;
+        const uint64_t r = stg * 64 + g * 16 + h * 8 + j;
         uint16_t val = 0;
         if (r < V) {
             if (tile < wtiles) {
@@ -117,8 +121,8 @@ __global__ void k_forward_frag(const float *__restrict__ W, uint64_t V, uint64_t
     uint16_t *dst = frag + id * 8;
 #pragma unroll
     for (int j = 0; j < 8; ++j) dst[j] = out[j];
-    // scales: one thread per (block, column) -- the first k-step of each block
-    if (tile < wtiles && g == 0 && (ks * kKStep) % kFlushRows == 0) {
+    // scales: one thread per (block, column) -- the first stage of each block
+    if (tile < wtiles && g == 0 && h == 0 && (stg * 64) % kBfFlushRows == 0) {
         const uint64_t c = (uint64_t)tile * 16 + n;
         if (c < ncol) {
             const uint32_t limb = (uint32_t)(c / K);
@@ -129,12 +133,12 @@ __global__ void k_forward_frag(const float *__restrict__ W, uint64_t V, uint64_t
 }
 
 hipError_t launch_forward_frag(const float *W, uint64_t V, uint64_t K, uint32_t L, const int *exps,
-                               uint32_t wtiles, uint32_t tiles, uint64_t nks, uint16_t *frag,
+                               uint32_t wtiles, uint32_t tiles, uint64_t nst, uint16_t *frag,
                                double *scales, hipStream_t stream) {
-    const uint64_t n = nks * tiles * 64;
+    const uint64_t n = nst * tiles * 128;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_forward_frag, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, W, V, K, L,
-                       exps, wtiles, tiles, nks, frag, scales);
+                       exps, wtiles, tiles, nst, frag, scales);
     return hipGetLastError();
 }
 

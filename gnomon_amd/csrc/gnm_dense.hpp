@@ -10,8 +10,9 @@ namespace gnm {
 // BF16 limbs (max |Q| = 2^(8L) - 2^(8(L-1))).
 hipError_t launch_block_exponent(const float *src, uint64_t rows, uint64_t ld, uint64_t cols,
                                  uint64_t block, uint32_t L, bool int8, int *out, hipStream_t stream);
+// split-BF16 B fragments of nst 64-row stages from dense W [V][K]
 hipError_t launch_forward_frag(const float *W, uint64_t V, uint64_t K, uint32_t L, const int *exps,
-                               uint32_t wtiles, uint32_t tiles, uint64_t nks, uint16_t *frag,
+                               uint32_t wtiles, uint32_t tiles, uint64_t nst, uint16_t *frag,
                                double *scales, hipStream_t stream);
 hipError_t launch_dense_program(const uint8_t *flip, uint64_t V, uint64_t Rpad, uint32_t *prog_var,
                                 uint32_t *flipmask, uint32_t *prog_cls, hipStream_t stream);

@@ -1,529 +1,12 @@
-// Split-int8 scoring kernel: the fused decode + contraction of
-// gnm_kernels.hip with weights written as signed 8-bit integer limbs and
-// v_mfma_i32_16x16x64_i8 (64 variants per MFMA, exact int32 accumulation).
-//
-// Per (scale block of kI8FlushRows = 65536 program rows, weight column) the
-// limb unit is a power of two 2^e with max|w| <= 127 * 256^(L-1) * 2^e;
-// w / 2^e is rounded to an integer Q and written in L signed base-256
-// digits, every digit in [-128, 127].  Dosages are 0/1/2, so each product is
-// an exact small integer and an int32 block sum is exact (65536 rows * 2 *
-// 128 = 2^24 < 2^31); blocks are flushed to FP64 with their scale.  L = 3
-// represents 23 bits, L = 4 31 bits of the block maximum.
-//
-// Data path, per workgroup of WAVES waves (64 people per wave, 16*WAVES
-// packed bytes of every row: 8 waves for 3-tile groups, so a workgroup reads
-// whole 128-byte lines of each row, else 4) and per stage of 64 program rows
-// (one k-step):
-//   * each thread loads 4 rows x 4 packed bytes (one dword per row;
-//     consecutive threads read consecutive dwords of a row), transposes the
-//     4x4 bytes in registers and writes 4 dwords into a column-major LDS tile
-//     [byte column][64 rows] (column pitch 80 bytes, row blocks swizzled per
-//     column so writes and 128-bit reads avoid bank conflicts);
-//   * one extra tile column holds a per-row selector byte: 4 for rows that
-//     read the A1 allele (flipped), 0 otherwise, and 0 for rows outside the
-//     split (whose data bytes are zero);
-//   * lane (i, g) of wave w reads column 16w + i, rows 16g..16g+15 as one
-//     128-bit LDS read: the packed codes of people 4(16w+i)+t, t = 0..3, for
-//     16 variants.  Person t's codes are isolated per byte and OR-ed with the
-//     row selector byte; one v_perm_b32 then maps (flip, code) to the i8
-//     dosage, another to the missing indicator:
-//        code 00 01 10 11  ->  dose 0 0 1 2 (A2 row) or 2 0 1 0 (A1 row),
-//                               missing 0 1 0 0;
-//   * M-tile t of the wave holds people 64w + 4i + t; B fragments (16 i8 per
-//     lane, the MFMA B operand) go from memory to LDS by buffer LDS DMA,
-//     without passing through registers.
-// Loads of a batch of D stages are issued together (buffer loads: one
-// uniform descriptor per stage and a 32-bit lane offset, so no per-lane
-// 64-bit addresses stay live), stored, then consumed behind LDS-only
-// barriers.  The scale-block flush sits after the batch loop, so the
-// accumulators carried by that loop stay in accumulation registers.
-#include "gnm_internal.hpp"
-
-#include <utility>
+// Split-int8 path: the launcher of the fused decode + contraction kernel
+// (k_score_mx in gnm_score_mx.hpp with int8 limbs, v_mfma_i32_16x16x64_i8,
+// exact int32 block sums over scale blocks of kI8FlushRows = 65536 program
+// rows; L = 3 limbs represent 23 bits, L = 4 31 bits of the block maximum),
+// the device builders of the dense (differentiable) path's int8 fragments,
+// and the split-int8 adjoint.
+#include "gnm_score_mx.hpp"
 
 namespace gnm {
-
-typedef int i32x4_t __attribute__((ext_vector_type(4)));
-typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
-
-namespace {
-
-constexpr int kI8Stage = 64;      // rows per stage = one k-step of v_mfma_i32_16x16x64_i8
-constexpr int kColPitch = 80;     // LDS bytes per tile column (64 rows + 16 pad)
-constexpr int kI8RsrcWord3 = 0x00020000;   // raw buffer descriptor (gfx9): 32-bit data format
-
-// LDS dword of (tile column col, row quad m): the four 16-row blocks of a
-// column are permuted by an XOR that depends on col / 16, so the transposed
-// dword writes of a wave (16 columns x 4 row quads) fall on distinct banks,
-// while the 128-bit reads of a 16-row block stay contiguous and aligned.
-__device__ __forceinline__ int tile_dword(int col, int m) {
-    return col * (kColPitch / 4) + (m ^ (4 * ((col >> 4) & 3)));
-}
-
-// LDS written and read back by the same wave: LDS instructions of one wave
-// complete in order, so waiting for the writes is enough (no s_barrier).
-__device__ __forceinline__ void lds_wave_fence() {
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-}
-
-__device__ __forceinline__ void lds_barrier_i8() {
-    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
-
-// Barrier after LDS DMA (global_load_lds): the DMA completes on the vector
-// memory counter, so every wave waits for its own transfers first.
-__device__ __forceinline__ void dma_lds_barrier() {
-    // the same wait as an instruction the compiler sees (gfx9 encoding:
-    // vmcnt 0, expcnt 7, lgkmcnt 0), so it knows the DMA has landed and
-    // adds no waits for it to later LDS reads; the asm repeats it so that
-    // no memory operation can slip between the wait and the barrier
-    __builtin_amdgcn_s_waitcnt(0x0070);
-    asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
-
-// 4x4 byte transpose: r[k] byte b -> c[b] byte k.
-__device__ __forceinline__ void transpose4x4(const uint32_t (&r)[4], uint32_t (&c)[4]) {
-    const uint32_t lo01 = __builtin_amdgcn_perm(r[1], r[0], 0x05010400u);
-    const uint32_t hi01 = __builtin_amdgcn_perm(r[1], r[0], 0x07030602u);
-    const uint32_t lo23 = __builtin_amdgcn_perm(r[3], r[2], 0x05010400u);
-    const uint32_t hi23 = __builtin_amdgcn_perm(r[3], r[2], 0x07030602u);
-    c[0] = __builtin_amdgcn_perm(lo23, lo01, 0x05040100u);
-    c[1] = __builtin_amdgcn_perm(lo23, lo01, 0x07060302u);
-    c[2] = __builtin_amdgcn_perm(hi23, hi01, 0x05040100u);
-    c[3] = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
-}
-
-// selector bytes (code | flip<<2) -> i8 dosage / missing indicator
-__device__ __forceinline__ uint32_t dose_i8(uint32_t sel) {
-    return __builtin_amdgcn_perm(0x00010002u, 0x02010000u, sel);
-}
-__device__ __forceinline__ uint32_t miss_i8(uint32_t sel) {
-    return __builtin_amdgcn_perm(0x00000100u, 0x00000100u, sel);
-}
-
-// 16 bytes per lane from a buffer (lane offset voff) into LDS at
-// dst + 16 * lane (buffer_load_dwordx4 ... lds); dst must be wave-uniform.
-__device__ __forceinline__ void lds_dma_b128(__amdgpu_buffer_rsrc_t rsrc, void *dst, uint32_t voff) {
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (__attribute__((address_space(3))) void *)dst, 16, voff, 0, 0, 0);
-}
-
-}  // namespace
-
-template <int TMAX, int WAVES, int D, bool VEC, bool IDENT, int SETS>
-// The 3-tile, 2-stage instance is asked to fit four waves per SIMD (128
-// registers: a few values move to accumulation registers, none to memory).
-// SETS > 1: the waves form SETS sets over the same people, each set with its
-// own group of TMAX tiles, so one row read serves SETS groups (the first
-// set loads and stores the rows for all).
-__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX == 3 && D == 2 ? 4 : 1))) void k_score_i8(DevicePlan plan, const uint8_t *__restrict__ rows,
-                                                         uint64_t stride, uint64_t bpv, uint64_t v0, uint64_t p0,
-                                                         uint64_t p1, uint64_t split_base, uint64_t rps,
-                                                         uint32_t splits, uint32_t person_tiles,
-                                                         uint64_t num_people, uint32_t tile_begin0,
-                                                         uint32_t tile_end, uint32_t ngroups,
-                                                         double *__restrict__ part,
-                                                         uint32_t *__restrict__ cpart) {
-    static_assert(WAVES % SETS == 0, "whole sets of waves");
-    constexpr int PW = WAVES / SETS;                 // waves over distinct people
-    constexpr int kBytes = 16 * PW;                  // packed bytes per row per workgroup
-    constexpr int kCols = kBytes + 1;                // + the selector column
-    constexpr int kTileWords = kCols * kColPitch / 4;
-    constexpr int kBT = SETS * TMAX;                 // B tiles per stage (all sets)
-    constexpr int kBW = (kBT + WAVES - 1) / WAVES;   // B tiles each wave fetches per stage
-    static_assert(kI8FlushRows % (D * kI8Stage) == 0, "scale blocks must hold whole batches");
-    // one LDS tile per stage of a batch: all D stages are stored, then one
-    // barrier, then all D are consumed (two barriers per batch of D stages)
-    __shared__ uint32_t tile[D][kTileWords];
-    // B fragments are double-buffered: the next batch's DMA lands in one
-    // half while this batch's MFMAs read the other
-    __shared__ u32x4_t btile[2][D][kBT * 64];
-
-    // Workgroups are dispatched round-robin over the 8 XCDs (L mod 8).  One
-    // launch covers every tile group: the groups of one (split, person tile)
-    // are consecutive slots of the same XCD, so they read the same .bed rows
-    // through that XCD's L2 at about the same time, and every workgroup of a
-    // split stays on one XCD (its B fragments are shared through that L2).
-    const uint32_t L = blockIdx.x;
-    const uint32_t xcd = L & 7u, slot = L >> 3;
-    const uint32_t group = slot % ngroups, rest = slot / ngroups;
-    const uint32_t s = xcd + 8u * (rest / person_tiles);
-    const uint32_t ptile = rest % person_tiles;
-    if (s >= splits) return;
-    const uint32_t gtile0 = tile_begin0 + group * kBT;   // first tile of the workgroup's sets
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int pwave = wave % PW, set = wave / PW;
-    // the first set loads and stores the rows of every stage
-    const bool loader = SETS == 1 || wave < PW;
-    const uint32_t tile_begin = gtile0 + (uint32_t)set * TMAX;
-    const uint32_t ntile = tile_begin >= tile_end ? 0u
-                           : (tile_end - tile_begin < (uint32_t)TMAX ? tile_end - tile_begin : (uint32_t)TMAX);
-    const int li = lane & 15, lg = lane >> 4;
-    const uint64_t col0 = (uint64_t)ptile * kBytes;
-    const uint64_t pw0 = (uint64_t)ptile * (64 * PW) + (uint64_t)pwave * 64;
-    const uint64_t sb = split_base + (uint64_t)s * rps;
-    const uint64_t r_begin = p0 > sb ? p0 : sb;
-    const uint64_t se = sb + rps;
-    const uint64_t r_end = p1 < se ? p1 : se;
-    const uint32_t wt = plan.wtiles;
-    const uint32_t tiles_total = plan.wtiles + plan.ctiles;
-    // weight tiles of this group (the first nw tiles); the others count missing calls
-    const uint32_t nw = tile_begin < wt ? ((wt - tile_begin) < ntile ? (wt - tile_begin) : ntile) : 0;
-    // load role: rows 4rg..4rg+3, packed bytes 4c4..4c4+3 of the workgroup's span
-    // (consecutive threads take consecutive dwords of a row: each load
-    // instruction reads 64 contiguous bytes of each of its rows)
-    const int rg = tid / (4 * PW), c4 = tid % (4 * PW);
-    const uint64_t ld_col = col0 + 4 * (uint64_t)c4;
-    const bool ld_any = ld_col < bpv;
-    // Every stage issues the same loads on every path (one dword per row):
-    // past the row end the dword is read at the last full dword of the row
-    // (bpv >= 4 on the vector path) and shifted / masked back at store time.
-    const uint64_t ld_colc = ld_col + 4 <= bpv ? ld_col : bpv - 4;
-    const uint32_t col_shift = ld_any ? (uint32_t)(ld_col - ld_colc) * 8u : 0u;
-    const uint32_t col_mask = ld_col + 4 <= bpv ? 0xFFFFFFFFu
-                              : (ld_any ? (1u << (8u * (uint32_t)(bpv - ld_col))) - 1u : 0u);
-    const u32x4_t *__restrict__ bfrag = (const u32x4_t *)plan.bfrag;
-
-    uint32_t ra[D][4];      // row dwords of each stage in flight
-    uint32_t rs[D];         // selector bytes of rows 4rg..4rg+3 (threads with c4 == 0)
-    uint32_t rk[D];         // keep mask bits 0..3 for rows 4rg+k inside the split
-    bool rfull[D];          // whole stage inside the split (uniform): no masking
-    // Fast path: every row of the stage is inside [r_begin, r_end) of an
-    // identity block and the workgroup's byte span lies inside the row, so
-    // the four row loads are one per-thread pointer plus uniform offsets.
-    const bool span_full = VEC && col0 + kBytes <= bpv;
-    // per-lane byte offset of the fast path's first row dword from the
-    // stage's first row (the launcher keeps 64 rows x stride below 2^31)
-    const uint32_t row_voff = (uint32_t)(4 * (uint64_t)rg * stride + ld_col);
-
-    const uint64_t last_st = ((r_end - 1) / kI8Stage) * kI8Stage;
-    // Stages are numbered from st0, the split's first row aligned down to a
-    // batch; per-stage tests use these 32-bit indices (scalar compares, where
-    // 64-bit row compares take vector instructions):
-    //   nst      stages of the split
-    //   fast_lo, fast_hi   stages wholly inside [r_begin, r_end) (fast path)
-    //   last_i   the stage holding the split's last row
-    const uint64_t st0 = (r_begin / (D * kI8Stage)) * (D * kI8Stage);
-    const bool nonempty = r_begin < r_end;
-    const uint32_t nst = nonempty ? (uint32_t)((r_end - st0 + kI8Stage - 1) / kI8Stage) : 0u;
-    const uint32_t fast_lo = (uint32_t)((r_begin - st0 + kI8Stage - 1) / kI8Stage);
-    const uint32_t fast_hi = (IDENT && span_full && nonempty) ? (uint32_t)((r_end - st0) / kI8Stage) : 0u;
-    const uint32_t last_i = nonempty ? (uint32_t)((last_st - st0) / kI8Stage) : 0u;
-    // B fragments: wave w fetches tiles w, w + WAVES, ... of the sets' kBT
-    // tiles by LDS DMA (lane l's 16 bytes land at btile[d][64 j + l]); tiles
-    // past tile_end are fetched clamped to the last tile and their products
-    // are never flushed
-    uint32_t boff[kBW];     // byte offsets within a stage's fragment block
-#pragma unroll
-    for (int m = 0; m < kBW; ++m) {
-        const uint32_t j = gtile0 + (uint32_t)(wave + m * WAVES);
-        const uint32_t jc = j < tile_end ? j : tile_end - 1;
-        boff[m] = (jc * 64 + lane) * 16;
-    }
-    // fragment block of stage index si (clamped to last_i by the caller)
-    const u32x4_t *__restrict__ bfrag_st0 = bfrag + (st0 / kI8Stage) * tiles_total * 64;
-    const uint32_t bstep = tiles_total * 64;
-    auto fetch_b = [&](uint32_t par, int d, uint32_t si) __attribute__((always_inline)) {
-        const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
-            (void *)(bfrag_st0 + (uint64_t)si * bstep), 0, 0x7FFFFFFF, kI8RsrcWord3);
-#pragma unroll
-        for (int m = 0; m < kBW; ++m) {
-            const int j = wave + m * WAVES;
-            if (kBW * WAVES == kBT || j < kBT)
-                lds_dma_b128(rb, &btile[par][d][j * 64], boff[m]);
-        }
-    };
-    auto load_row = [&](uint64_t var, uint64_t lcol) __attribute__((always_inline)) -> uint32_t {
-        const uint8_t *src = rows + var * stride;
-        if (VEC) return *(const uint32_t *)(src + (lcol + 4 <= bpv ? lcol : bpv - 4));
-        uint32_t x = 0;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const uint64_t o = lcol + b;
-            const uint32_t byte = src[o < bpv ? o : bpv - 1];
-            x |= (o < bpv ? byte : 0u) << (8 * b);
-        }
-        return x;
-    };
-    auto issue = [&](uint32_t par, int d, uint32_t si) __attribute__((always_inline)) {
-        const uint64_t st_req = st0 + (uint64_t)si * kI8Stage;
-        if (si >= fast_lo && si < fast_hi) {   // IDENT && span_full, stage inside the split
-            // buffer loads: a uniform descriptor per stage, one 32-bit lane
-            // offset, and the row step as the scalar offset
-            const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
-                (void *)(rows + (st_req - p0) * stride), 0, 0x7FFFFFFF, kI8RsrcWord3);
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                ra[d][k] = __builtin_amdgcn_raw_buffer_load_b32(rr, row_voff, (uint32_t)(k * stride), 0);
-            rk[d] = 0xFu;
-            rfull[d] = true;
-            const __amdgpu_buffer_rsrc_t rf = __builtin_amdgcn_make_buffer_rsrc(
-                (void *)(plan.flipbyte + st_req), 0, 0x7FFFFFFF, kI8RsrcWord3);
-            rs[d] = __builtin_amdgcn_raw_buffer_load_b32(rf, 4u * (uint32_t)rg, 0, 0);
-            return;
-        }
-        // Slow path (split edges, partial spans, non-identity rows): its
-        // per-lane inputs pass through an opaque copy, so what it derives
-        // from them is computed here and not kept live across the loop.
-        uint32_t rg4 = 4u * (uint32_t)rg;
-        uint64_t lcol = ld_col;
-        asm volatile("" : "+v"(rg4), "+v"(lcol));
-        const uint64_t st = st_req < last_st ? st_req : last_st;
-        const bool same = st == st_req;
-        uint32_t keep = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint64_t r = st + rg4 + k;
-            const bool ok = same && r >= r_begin && r < r_end && lcol < bpv;
-            keep |= ok ? (1u << k) : 0u;
-            const uint64_t rc = r < r_begin ? r_begin : (r >= r_end ? r_end - 1 : r);
-            const uint64_t var = IDENT ? (rc - p0) : (uint64_t)(plan.prog_var[rc] - (uint32_t)v0);
-            ra[d][k] = load_row(var, lcol);
-        }
-        rk[d] = keep;
-        rfull[d] = false;
-        rs[d] = *(const uint32_t *)(plan.flipbyte + st + rg4);   // selector bytes of rows st+4rg..+3 (c4 == 0)
-    };
-    auto store = [&](int buf, int d) __attribute__((always_inline)) {
-        uint32_t r[4], c[4];
-        const uint32_t keep = rk[d];
-        if (rfull[d]) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) r[k] = ra[d][k];
-        } else {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const uint32_t x = VEC ? (ra[d][k] >> col_shift) & col_mask : ra[d][k];
-                r[k] = ((keep >> k) & 1u) ? x : 0u;
-            }
-        }
-        transpose4x4(r, c);
-        uint32_t *t = tile[buf];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) t[tile_dword(4 * c4 + b, rg)] = c[b];
-        if (c4 == 0) {
-            // selector bytes, zero for rows outside the split
-            uint32_t sel = rs[d];
-            if (!rfull[d]) {
-                const uint32_t km = ((keep & 1u) ? 0xFFu : 0u) | ((keep & 2u) ? 0xFF00u : 0u) |
-                                    ((keep & 4u) ? 0xFF0000u : 0u) | ((keep & 8u) ? 0xFF000000u : 0u);
-                sel &= km;
-            }
-            t[tile_dword(kBytes, rg)] = sel;
-        }
-    };
-
-    typedef i32x4_t Acc[4][TMAX];   // int32 block sums [person tile t][tile j]
-    // zero from an opaque copy of 0: the initialisation stays in the branch
-    // that uses the set instead of being hoisted above the layout dispatch
-    auto zero_acc = [&](Acc &acc) __attribute__((always_inline)) {
-        int z = 0;
-        asm volatile("" : "+v"(z));
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-#pragma unroll
-            for (int j = 0; j < TMAX; ++j) acc[t][j] = i32x4_t{z, z, z, z};
-    };
-
-    // Block flush: acc[t][j][q] is the int32 block sum of person
-    // pw0 + 16 lg + 4 q + t (MFMA output row 4 lg + q of M-tile t) and tile
-    // column li.  The wave stages one tile through its own LDS buffer so
-    // that each lane then owns one column for 16 people, and adds the values
-    // to the split partials in FP64 (weight tiles, times the block scale) or
-    // as counts.
-    // the staging buffer shares the .bed tiles' LDS when they are large
-    // enough (no tile is in use while a block is flushed)
-    constexpr bool kStageInTile = D * kTileWords >= WAVES * 64 * 16;
-    __shared__ int stage_own[kStageInTile ? 1 : WAVES * 64 * 16];
-    auto flush = [&](Acc &acc, uint64_t blk, bool first, bool empty) __attribute__((always_inline)) {
-        // opaque to the optimiser, so the addresses below are formed here
-        // rather than hoisted out of the row loop into live registers
-        uint64_t prow = (uint64_t)s * num_people + pw0;
-        uint32_t ncol = plan.ncol, nclass = plan.nclass, tb = tile_begin, lane_c = (uint32_t)li;
-        asm volatile("" : "+v"(prow), "+v"(ncol), "+v"(nclass), "+v"(tb), "+v"(lane_c));
-        ncol = __builtin_amdgcn_readfirstlane(ncol);
-        nclass = __builtin_amdgcn_readfirstlane(nclass);
-        tb = __builtin_amdgcn_readfirstlane(tb);
-        int *sbuf = (kStageInTile ? (int *)&tile[0][0] : stage_own) + wave * (64 * 16);
-        auto flush_tile = [&](auto jc) __attribute__((always_inline)) {
-            constexpr int j = decltype(jc)::value;
-            __builtin_amdgcn_sched_barrier(0);
-            // keeps this tile's sums in accumulation registers up to here,
-            // so only 16 of them are copied out at a time
-#pragma unroll
-            for (int t = 0; t < 4; ++t) asm volatile("" : "+a"(acc[t][j]));
-            lds_wave_fence();    // the previous tile's reads of sbuf are done
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) sbuf[(16 * lg + 4 * q + t) * 16 + li] = acc[t][j][q];
-                acc[t][j] = i32x4_t{0, 0, 0, 0};
-            }
-            lds_wave_fence();
-            const uint32_t tl = tb + j;
-            const bool wtile = tl < wt;
-            const uint32_t c = (wtile ? tl : tl - wt) * 16 + lane_c;
-            const uint32_t nc = wtile ? ncol : nclass;
-            if ((uint32_t)j < ntile && c < nc) {
-                const double sc = (wtile && !empty) ? plan.scales[blk * ncol + c] : 0.0;
-#pragma unroll 1
-                for (int i = 0; i < 16; ++i) {
-                    const uint32_t pl = 4 * i + lg;  // person of the wave, 0..63
-                    if (pw0 + pl >= num_people) continue;
-                    const int v = sbuf[pl * 16 + lane_c];
-                    const uint64_t row = prow + pl;
-                    if (wtile) {
-                        double *dd = part + row * ncol + c;
-                        const double x = (double)v * sc;
-                        *dd = first ? x : *dd + x;
-                    } else {
-                        uint32_t *dd = cpart + row * nclass + c;
-                        const uint32_t x = (uint32_t)v;
-                        *dd = first ? x : *dd + x;
-                    }
-                }
-            }
-            __builtin_amdgcn_sched_barrier(0);
-        };
-        [&]<int... J>(std::integer_sequence<int, J...>) __attribute__((always_inline)) {
-            (flush_tile(std::integral_constant<int, J>{}), ...);
-        }(std::make_integer_sequence<int, TMAX>{});
-        // staging inside the .bed tiles: no wave may store the next batch's
-        // tiles while another still reads its staged sums
-        if constexpr (kStageInTile) lds_barrier_i8();
-    };
-
-    // compute(nwc, par, buf): one stage.  With a compile-time layout
-    // (nwc >= 0: the first nwc tiles are weight tiles, the others count
-    // tiles) the stage is straight-line code, so the scheduler can overlap
-    // one stage's decode with the previous stage's MFMAs; tile slots past
-    // ntile are computed on a duplicated fragment and never flushed.  With
-    // nwc < 0 the layout is read at run time (per-tile branches).
-    auto compute = [&](Acc &acc, auto nwc, uint32_t par, int buf) __attribute__((always_inline)) {
-        constexpr int NW = decltype(nwc)::value;
-        const uint32_t *t = tile[buf];
-        const u32x4_t X = *(const u32x4_t *)&t[tile_dword(pwave * 16 + li, 4 * lg)];
-        const u32x4_t F = *(const u32x4_t *)&t[tile_dword(kBytes, 4 * lg)];
-        u32x4_t bv[TMAX];
-#pragma unroll
-        for (int j = 0; j < TMAX; ++j) bv[j] = btile[par][buf][(set * TMAX + j) * 64 + lane];
-        // person tile by person tile: only one tile's selectors and operands
-        // are live at a time
-#pragma unroll
-        for (int tt = 0; tt < 4; ++tt) {
-            uint32_t sel[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) sel[q] = ((X[q] >> (2 * tt)) & 0x03030303u) | F[q];
-            i32x4_t A, M;
-            if constexpr (NW != 0) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) A[q] = (int)dose_i8(sel[q]);
-            }
-            auto mfma = [&](int j, const i32x4_t &op) __attribute__((always_inline)) {
-                const i32x4_t B = i32x4_t{(int)bv[j][0], (int)bv[j][1], (int)bv[j][2], (int)bv[j][3]};
-                acc[tt][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(op, B, acc[tt][j], 0, 0, 0);
-            };
-            if constexpr (NW >= 0) {
-                if constexpr (NW < TMAX) {
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) M[q] = (int)miss_i8(sel[q]);
-                }
-                // a wave issuing its MFMAs goes ahead of waves decoding, so
-                // the matrix pipe is fed while they run (3% at 64 scores)
-                __builtin_amdgcn_s_setprio(1);
-#pragma unroll
-                for (int j = 0; j < TMAX; ++j) mfma(j, j < NW ? A : M);
-                __builtin_amdgcn_s_setprio(0);
-            } else {
-#pragma unroll
-                for (int j = 0; j < TMAX; ++j)
-                    if ((uint32_t)j < nw) mfma(j, A);
-                if (nw < ntile) {
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) M[q] = (int)miss_i8(sel[q]);
-#pragma unroll
-                    for (int j = 0; j < TMAX; ++j)
-                        if ((uint32_t)j >= nw && (uint32_t)j < ntile) mfma(j, M);
-                }
-            }
-        }
-    };
-
-    if (r_begin >= r_end) {
-        Acc acc;
-        zero_acc(acc);
-        flush(acc, 0, true, true);
-        return;
-    }
-    // Stages are processed in batches of D, software-pipelined: a batch's
-    // row loads and B-fragment DMA are issued before the previous batch is
-    // computed, so their latency overlaps the MFMAs; a batch is stored to
-    // LDS only once its loads have landed.
-    auto issue_batch = [&](uint32_t par, uint32_t i) __attribute__((always_inline)) {
-        // stages past the split are loaded clamped and never stored
-        if (!loader) return;
-        [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
-            (issue(par, I, i + I), ...);
-        }(std::make_integer_sequence<int, D>{});
-    };
-    auto fetch_batch = [&](uint32_t par, uint32_t i) __attribute__((always_inline)) {
-        [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
-            (fetch_b(par, I, i + I < last_i ? i + I : last_i), ...);
-        }(std::make_integer_sequence<int, D>{});
-    };
-    auto main_loop = [&](auto nwc) __attribute__((always_inline)) {
-        Acc acc;   // one accumulator set per copy of the loop
-        zero_acc(acc);
-        uint32_t par = 0;
-        uint32_t i0 = 0;
-        issue_batch(par, i0);
-        fetch_batch(par, i0);
-        // Outer loop over scale blocks, inner loop over batches of the block:
-        // the flush sits after the inner loop on every path, so the
-        // accumulators carried by the inner loop are only ever MFMA results and
-        // stay in accumulation registers.
-        bool first = true;
-        do {   // nst > 0
-            const uint64_t blk = (st0 + (uint64_t)i0 * kI8Stage) / kI8FlushRows;
-            const uint64_t blk_end = ((blk + 1) * kI8FlushRows - st0) / kI8Stage;   // a multiple of D
-            const uint32_t i_end = blk_end < nst ? (uint32_t)blk_end : nst;
-            do {   // i0 < i_end on entry: the block holds stage i0
-                if (loader) {
-                    [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
-                        ((i0 + I < nst ? (store(I, I), 0) : 0), ...);
-                    }(std::make_integer_sequence<int, D>{});
-                }
-                dma_lds_barrier();   // this batch's B fragments landed and its tiles are stored
-                issue_batch(par ^ 1u, i0 + D);
-                [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
-                    ((i0 + I < nst ? (compute(acc, nwc, par, I), 0) : 0), ...);
-                }(std::make_integer_sequence<int, D>{});
-                // after this wave's last read of the batch's B fragments: the
-                // DMA into the other half never overlaps an LDS read in flight
-                fetch_batch(par ^ 1u, i0 + D);
-                lds_barrier_i8();   // every wave is done with the tiles before the next batch stores
-                par ^= 1u;
-                i0 += D;
-            } while (i0 < i_end);
-            flush(acc, blk, first, false);
-            first = false;
-        } while (i0 < nst);
-    };
-    // One copy of the loop per group layout on the common path (dword row
-    // loads, identity rows): nw is fixed per workgroup, so the choice is made
-    // once, and no accumulator value is merged across the copies.
-    if constexpr (VEC && IDENT) {
-        [&]<int... N>(std::integer_sequence<int, N...>) __attribute__((always_inline)) {
-            ((nw == (uint32_t)N ? (main_loop(std::integral_constant<int, N>{}), 0) : 0), ...);
-        }(std::make_integer_sequence<int, TMAX + 1>{});
-    } else {
-        main_loop(std::integral_constant<int, -1>{});
-    }
-    // the last, unused batch's DMA must land before the workgroup's LDS is released
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-}
 
 // Rough cost of one group pass per wave and stage, in cycles: decode VALU
 // (~60 instructions) plus 4 MFMA of 16 cycles per tile slot.
@@ -590,7 +73,7 @@ hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t
     const uint32_t pt = (uint32_t)((num_people + people_per_wg - 1) / people_per_wg);
     const uint64_t wgs = (uint64_t)pt * splits * ngroups;   /* splits is a multiple of 8 */
 #define GNM_LAUNCH_I8(TM, WV, DD, VC, ID, SE)                                                                \
-    hipLaunchKernelGGL((k_score_i8<TM, WV, DD, VC, ID, SE>), dim3((unsigned)wgs), dim3(64 * WV), 0, stream,     \
+    hipLaunchKernelGGL((k_score_mx<TM, WV, DD, VC, ID, SE, false>), dim3((unsigned)wgs), dim3(64 * WV), 0, stream,     \
                        plan, rows, stride, bpv, v0, p0, p1, split_base, rows_per_split, splits, pt, num_people, \
                        tile_begin, tile_end, ngroups, part, cpart)
 #define GNM_LAUNCH_I8_VI(TM, WV, DD, SE)                                                                     \
@@ -932,15 +415,19 @@ hipError_t launch_adjoint_i8(const uint8_t *rows, uint64_t stride, uint64_t V, u
     return hipGetLastError();
 }
 
-// per-row selector bytes (4 = flipped row) from a flip byte array
-__global__ void k_flip_bytes(const uint8_t *__restrict__ flip, uint64_t V, uint64_t Rpad, uint8_t *__restrict__ out) {
+// per-row selector bytes (value for a flipped row, 0 otherwise) from a flip
+// byte array
+__global__ void k_flip_bytes(const uint8_t *__restrict__ flip, uint64_t V, uint64_t Rpad, uint8_t value,
+                             uint8_t *__restrict__ out) {
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < Rpad) out[r] = (flip && r < V && flip[r]) ? 4u : 0u;
+    if (r < Rpad) out[r] = (flip && r < V && flip[r]) ? value : (uint8_t)0;
 }
 
-hipError_t launch_flip_bytes(const uint8_t *flip, uint64_t V, uint64_t Rpad, uint8_t *out, hipStream_t stream) {
+hipError_t launch_flip_bytes(const uint8_t *flip, uint64_t V, uint64_t Rpad, uint8_t value, uint8_t *out,
+                             hipStream_t stream) {
     if (Rpad == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_flip_bytes, dim3((unsigned)((Rpad + 255) / 256)), dim3(256), 0, stream, flip, V, Rpad, out);
+    hipLaunchKernelGGL(k_flip_bytes, dim3((unsigned)((Rpad + 255) / 256)), dim3(256), 0, stream, flip, V, Rpad,
+                       value, out);
     return hipGetLastError();
 }
 

@@ -9,24 +9,24 @@
 
 namespace gnm {
 
-// Program rows are grouped in k-steps of 32 (the reduction depth of one
-// v_mfma_f32_16x16x32_bf16) and in scale blocks of kFlushRows (the span over
-// which FP32 partial sums of integer-valued BF16 limbs stay exact:
-// 8192 rows * dosage 2 * |limb| 256 = 2^22 < 2^24).
-constexpr int kKStep = 32;
+// People blocks of the split-BF16 adjoint (FP32 partial sums of
+// integer-valued BF16 limbs stay exact: 8192 people * dosage 2 * |limb| 256
+// = 2^22 < 2^24).
 constexpr int kFlushRows = 8192;
-constexpr int kKStepsPerFlush = kFlushRows / kKStep;
 // The split-int8 path has its own, longer scale blocks: an int32 sum of
 // dosage (<= 2) x int8 limb (|.| <= 128) products stays exact up to 2^22
 // rows; 65536 rows (sums < 2^24) keep the FP64 read-modify-write flushes of
 // the partial sums rare.
 constexpr int kI8FlushRows = 65536;
-constexpr int flush_rows(bool int8) { return int8 ? kI8FlushRows : kFlushRows; }
+// Scale blocks of the split-BF16 scoring kernel: an FP32 sum of dosage (<= 2)
+// x BF16 limb digit (|.| <= 255) products stays an exact integer below 2^24
+// over 32768 rows (32768 * 510 = 16711680).  (The split-BF16 adjoint keeps
+// kFlushRows-person blocks.)
+constexpr int kBfFlushRows = 32768;
+constexpr int flush_rows(bool int8) { return int8 ? kI8FlushRows : kBfFlushRows; }
 constexpr int kLimbBits = 8;
 constexpr int kFp32ScoreChunk = 8;     // scores per pass of the FP32 kernel
 constexpr int kFp32MaxClasses = 4;     // membership classes counted in-kernel by the FP32 kernel
-constexpr int kMfmaMaxTiles = 4;       // 16-column tiles per pass of the MFMA kernel (two waves per SIMD;
-                                        // 8-tile passes fit one and took 98 ms against 76 ms at 100k x 1M x 64)
 
 // Signed base-256 digits of an integer with |Q| <= 2^(8L) - 2^(8(L-1)):
 // low digits in [-128, 127], top digit in [-255, 255]; all exact in BF16.
@@ -87,6 +87,8 @@ struct DevicePlan {
     uint32_t wtiles;           // ceil(ncol/16)
     uint32_t ctiles;           // ceil(nclass/16)
     uint32_t nclass;           // membership classes (distinct CSR column supports)
+    uint32_t vcount;           // 1: one class and no count tiles -- the MFMA kernels count missing
+                               // calls on the vector ALUs (selector bit 7 marks count rows)
     const uint32_t *score_class; // [K]
 };
 
@@ -100,21 +102,11 @@ hipError_t launch_unpack(const uint8_t *rows, uint64_t stride, uint64_t num_rows
 hipError_t launch_compact(const uint8_t *rows, uint64_t stride, uint64_t num_rows,
                           const uint32_t *out_to_fam, uint64_t num_people, uint8_t *out,
                           uint64_t out_stride, hipStream_t stream);
-// Number of variant splits (a multiple of 8, 8..64) that fills the device.
-uint32_t mfma_splits(uint64_t num_people, uint32_t tiles);
 uint32_t fp32_splits(uint64_t num_people, uint64_t num_scores);
 hipError_t launch_score_fp32(const DevicePlan &plan, const uint8_t *rows, uint64_t stride,
                              uint64_t bpv, uint64_t v0, uint64_t p0, uint64_t p1,
                              uint64_t num_people, uint64_t num_scores, uint32_t splits,
                              bool count_in_kernel, double *part, uint32_t *cpart, hipStream_t stream);
-// identity: program rows [p0, p1) are the consecutive variants [v0, v0+p1-p0).
-// rows must be 16-byte aligned with a 16-byte multiple stride for the
-// 128-bit staging loads (else a byte-load variant runs)
-hipError_t launch_score_mfma(const DevicePlan &plan, const uint8_t *rows, uint64_t stride,
-                             uint64_t bpv, uint64_t v0, uint64_t p0, uint64_t p1,
-                             uint64_t num_people, uint32_t splits,
-                             uint32_t tile_begin, uint32_t tile_end, bool identity, double *part,
-                             uint32_t *cpart, hipStream_t stream);
 // Split-int8 scoring (gnm_i8.hip); rows must be 4-byte aligned with a
 // 4-byte multiple stride below 2^25 for the dword staging loads (else byte
 // loads).  One launch covers every tile group of [tile_begin, tile_end)
@@ -128,6 +120,16 @@ hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t
                            uint32_t tiles_per_pass = 0, uint32_t batch_stages = 0);
 // variant splits of the int8 kernel for num_people and ntiles tiles
 uint32_t i8_splits(uint64_t num_people, uint32_t ntiles);
+// Split-BF16 scoring (gnm_bf16.hip): the same kernel with BF16 limbs (B
+// fragments [64-row stage][tile][k-step h][lane][8 bf16], rows 16 g + 8 h + j
+// of the stage for lane (n, g)), the same row and stride conditions, one
+// launch over every tile group of [tile_begin, tile_end) (groups of
+// tiles_per_pass tiles: 0 = default, else 3, 5, 6, 8 or 9).
+hipError_t launch_score_bf16(const DevicePlan &plan, const uint8_t *rows, uint64_t stride, uint64_t bpv, uint64_t v0,
+                             uint64_t p0, uint64_t p1, uint64_t num_people, uint32_t splits, uint32_t tile_begin,
+                             uint32_t tile_end, bool identity, double *part, uint32_t *cpart, hipStream_t stream,
+                             uint32_t tiles_per_pass = 0);
+uint32_t bf16_splits(uint64_t num_people, uint32_t ntiles);
 hipError_t launch_forward_frag_i8(const float *W, uint64_t V, uint64_t K, uint32_t L, const int *exps,
                                   uint32_t wtiles, uint32_t tiles, uint64_t nks, uint32_t *frag, double *scales,
                                   hipStream_t stream);
@@ -140,7 +142,11 @@ hipError_t launch_adjoint_frag_i8(const float *R, uint64_t N, uint64_t K, uint64
 hipError_t launch_adjoint_i8(const uint8_t *rows, uint64_t stride, uint64_t V, uint64_t N, const uint8_t *flip,
                              const uint32_t *frag, const double *scales, uint32_t tiles, uint64_t K, uint64_t Kp,
                              uint32_t L, double *ws, float *dW, hipStream_t stream);
-hipError_t launch_flip_bytes(const uint8_t *flip, uint64_t V, uint64_t Rpad, uint8_t *out, hipStream_t stream);
+// per-row selector bytes of the scoring kernel: kFlipSelI8 (int8 path) or
+// kFlipSelBf (BF16 path) for flipped rows
+constexpr uint8_t kFlipSelI8 = 4, kFlipSelBf = 0x55;
+hipError_t launch_flip_bytes(const uint8_t *flip, uint64_t V, uint64_t Rpad, uint8_t value, uint8_t *out,
+                             hipStream_t stream);
 constexpr int kI8KStep = 64;           // rows per k-step of the int8 kernel
 
 // Device build of the weight plan (gnm_plan.hip).  wrow: f64 [R][K]
@@ -154,8 +160,9 @@ hipError_t launch_plan_rows(const uint64_t *off, const uint32_t *cols, const flo
 // and the limb scales [block][L*K]
 hipError_t launch_plan_exps(const double *wrow, uint64_t R, uint64_t K, uint64_t fr, double qmax, uint32_t L,
                             uint64_t ncol, uint64_t nblk, int *uexp, double *scales, hipStream_t stream);
-// B fragments of nks k-steps (int8: 64 rows, [ks][tile][lane][16 i8];
-// BF16: 32 rows, [ks][tile][lane][8 bf16])
+// B fragments of nks 64-row stages (int8: one k-step each,
+// [stage][tile][lane][16 i8]; BF16: two k-steps each,
+// [stage][tile][k-step][lane][8 bf16])
 hipError_t launch_plan_frag(bool int8, const double *wrow, uint64_t R, uint64_t K, const int *uexp, uint64_t fr,
                             uint32_t L, uint32_t ncol, uint32_t wtiles, uint32_t tiles, uint64_t nks,
                             const uint8_t *count_row, const uint32_t *rowcls, uint32_t cw, uint32_t nclass,

@@ -488,335 +488,6 @@ hipError_t launch_score_fp32(const DevicePlan &plan, const uint8_t *rows, uint64
     return hipGetLastError();
 }
 
-// ---------------------------------------------------------------------------
-// Split-BF16 MFMA path (v_mfma_f32_16x16x32_bf16), genotype unpack fused.
-//
-// Workgroup: 4 waves x 64 people.  Lane (i = lane%16, g = lane/16) owns the
-// packed byte column i of its wave (people 4i..4i+3) and the eight rows
-// 8g..8g+7 of each 32-row k-step.  The four people of the byte are four
-// M-tiles (t = 0..3): M-tile t, row i is person wave_base + 4i + t.
-// Decode per pair of rows: P = b0 | b1<<16, p = P*0x0101 = [b0,b0,b1,b1];
-// selector x = ((p >> 2t) & 0x03030303) | 0x04000400 = [c0, c0+4, c1, c1+4];
-// v_perm over the 8-byte table {lo bytes 00 00 80 00 | hi bytes 00 00 3F 40}
-// yields BF16 {0, 0, 1.0, 2.0} for codes {00, 01, 10, 11}; the table
-// {00 80 00 00 | 00 3F 00 00} yields the missing indicator for counting.
-// B fragments are pre-arranged on the host in exactly the lane order of the
-// instruction, so every lane issues one 16-byte load per 16-column tile.
-// Integer-valued limbs make each FP32 partial sum exact inside a scale block
-// of kFlushRows rows; blocks are scaled and combined in FP64.
-// ---------------------------------------------------------------------------
-// selector [c0, c0+4, c1, c1+4] from the two codes in bits 0-1 of bytes 0
-// and 2 (bytes 1 and 3 hold copies): (y & 0x03030303) | 0x04000400 as one
-// v_and_or_b32 (the compiler otherwise emits an and plus an or)
-__device__ __forceinline__ uint32_t sel_pair(uint32_t y) {
-    uint32_t x;
-    const uint32_t m = 0x03030303u, c = 0x04000400u;
-    asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(x) : "v"(y), "v"(m), "v"(c));
-    return x;
-}
-__device__ __forceinline__ uint32_t dose_pair(uint32_t x) {
-    return __builtin_amdgcn_perm(0x403F0000u, 0x00800000u, x);
-}
-__device__ __forceinline__ uint32_t miss_pair(uint32_t x) {
-    return __builtin_amdgcn_perm(0x00003F00u, 0x00008000u, x);
-}
-
-// Workgroup barrier for LDS traffic only: waits for this wave's LDS
-// operations and synchronises the waves, without the release fence of
-// __syncthreads() that would also drain every outstanding global load (the
-// prefetch ring must stay in flight across the barrier).
-__device__ __forceinline__ void lds_barrier() {
-    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
-
-union Frag {
-    bf16x8_t v;
-    uint32_t u[4];
-};
-
-__device__ __forceinline__ uint32_t flip_word(uint32_t b) {
-    return ~(((b & 0x55555555u) << 1) | ((b >> 1) & 0x55555555u));
-}
-
-// Workgroup of WAVES waves x 64 people = 16*WAVES packed bytes of every row.
-// Rows and B fragments move HBM/L2 -> registers -> LDS in stages of 64 rows
-// (two k-steps).  A ring of D stages is in flight per thread: at stage i the
-// loads of stage i+D-1 are issued, stage i (loaded D-1 stages earlier) is
-// written to one of two LDS buffers -- with the row's effect-allele flip
-// applied -- and, after one barrier, feeds the MFMAs.  Thread t loads the
-// 16-byte chunk (row t/WAVES, chunk t%WAVES) of every stage with one 128-bit
-// load; the LDS tile pitch is 16*WAVES+4 bytes so the eight rows a lane reads
-// sit on distinct banks.  Blocks whose program rows are consecutive variants
-// (no mixed-flip variant) skip the row-index indirection, so no load depends
-// on another load and the ring stays full.
-template <int TMAX, int WAVES, int D, bool VEC, bool IDENT>
-__global__ __launch_bounds__(64 * WAVES) void k_score_mfma(DevicePlan plan, const uint8_t *__restrict__ rows,
-                                                           uint64_t stride, uint64_t bpv, uint64_t v0,
-                                                           uint64_t p0, uint64_t p1, uint64_t split_base,
-                                                           uint64_t rps, uint32_t splits,
-                                                           uint32_t person_tiles, uint64_t num_people,
-                                                           uint32_t tile_begin, uint32_t ntile,
-                                                           double *__restrict__ part,
-                                                           uint32_t *__restrict__ cpart) {
-    constexpr int kBytes = 16 * WAVES;             // row bytes per workgroup
-    constexpr int kPitch = kBytes + 4;             // LDS row pitch
-    constexpr int kStage = 64;                     // rows per stage
-    constexpr int kNB = 2 * TMAX / WAVES;          // B chunks per thread per stage
-    static_assert(2 * TMAX % WAVES == 0, "B chunks must split evenly over the waves");
-    __shared__ uint32_t tile[2][kStage * kPitch / 4];
-    __shared__ u32x4_t btile[2][2 * TMAX * 64];
-    __shared__ float stage[WAVES][64 * 16];
-
-    // XCD-aware placement: workgroups are dispatched round-robin over the 8
-    // XCDs; every workgroup of one split lands on the same XCD so the split's
-    // B fragments are shared through that XCD's L2.
-    const uint32_t L = blockIdx.x;
-    const uint32_t xcd = L & 7u, slot = L >> 3;
-    const uint32_t s = xcd + 8u * (slot / person_tiles);
-    const uint32_t ptile = slot % person_tiles;
-    if (s >= splits) return;
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int li = lane & 15, lg = lane >> 4;
-    const uint64_t col0 = (uint64_t)ptile * kBytes;
-    const uint64_t pw0 = (uint64_t)ptile * (64 * WAVES) + (uint64_t)wave * 64;
-    const uint64_t sb = split_base + (uint64_t)s * rps;
-    const uint64_t r_begin = p0 > sb ? p0 : sb;
-    const uint64_t se = sb + rps;
-    const uint64_t r_end = p1 < se ? p1 : se;
-    const uint32_t wt = plan.wtiles;
-    const uint32_t tiles_total = plan.wtiles + plan.ctiles;
-    const uint32_t nw = tile_begin < wt ? ((wt - tile_begin) < ntile ? (wt - tile_begin) : ntile) : 0;
-    const bool need_miss = nw < ntile;
-    const int ld_row = tid / WAVES, ld_chunk = tid % WAVES;
-    const uint64_t ld_col = col0 + (uint64_t)ld_chunk * 16;
-    const bool ld_colok = ld_col < bpv;
-    const u32x4_t *__restrict__ bfrag = (const u32x4_t *)plan.bfrag;
-
-    u32x4_t rb[D];          // row chunk of each stage in flight
-    u32x4_t rB[D][kNB];     // B chunks of each stage in flight
-    uint32_t rw[D];         // flip-mask word of this thread's row (raw load)
-    uint32_t rf[D];         // bits 0-4: flip bit index, bit 5: row inside the split
-
-    // Every load is issued unconditionally (clamped to a valid address, the
-    // value discarded at store time) so the compiler can count outstanding
-    // loads exactly and keep D-1 stages in flight across the barriers.
-    // Stages strictly inside [r_begin, r_end) of an identity block take the
-    // fast path: per-thread pointers advanced by uniform strides.
-    const uint64_t last_st = ((r_end - 1) / kStage) * kStage;
-    // per-thread constants of the B-chunk addresses: chunk m is (h, j, l)
-    uint32_t boff[kNB];
-#pragma unroll
-    for (int m = 0; m < kNB; ++m) {
-        const int q = tid + m * 64 * WAVES;
-        const int h = q / (TMAX * 64), j = (q / 64) % TMAX, l = q % 64;
-        const uint32_t jc = (uint32_t)j < ntile ? (uint32_t)j : ntile - 1;  // tiles >= ntile are never read
-        boff[m] = (uint32_t)((h * tiles_total + tile_begin + jc) * 64 + l);
-    }
-    const uint8_t *bed_row0 = rows + (uint64_t)ld_row * stride + (ld_colok ? ld_col : 0);
-    const uint32_t rf_fast = (uint32_t)(ld_row & 31) | (ld_colok ? 32u : 0u);
-    auto issue = [&](int d, uint64_t st_req) __attribute__((always_inline)) {
-        if (IDENT && st_req >= r_begin && st_req + kStage <= r_end) {
-            rb[d] = *(const u32x4_t *)(bed_row0 + (st_req - p0) * stride);
-            rw[d] = plan.flipmask[(st_req >> 5) + (ld_row >> 5)];
-            rf[d] = rf_fast;
-            const u32x4_t *bs = bfrag + (st_req / kKStep) * tiles_total * 64;
-#pragma unroll
-            for (int m = 0; m < kNB; ++m) rB[d][m] = bs[boff[m]];
-            return;
-        }
-        const uint64_t st = st_req < last_st ? st_req : last_st;
-        const uint64_t r = st + ld_row;
-        const bool ok = (st_req == st) && (r >= r_begin) && (r < r_end) && ld_colok;
-        const uint64_t rc = r < r_begin ? r_begin : (r >= r_end ? r_end - 1 : r);
-        const uint64_t var = IDENT ? (rc - p0) : (uint64_t)(plan.prog_var[rc] - (uint32_t)v0);
-        const uint64_t cc = ld_colok ? ld_col : 0;
-        const uint8_t *src = rows + var * stride + cc;
-        u32x4_t v;
-        if (VEC) {
-            v = *(const u32x4_t *)src;
-        } else {
-#pragma unroll
-            for (int dd = 0; dd < 4; ++dd) {
-                uint32_t x = 0;
-#pragma unroll
-                for (int bb = 0; bb < 4; ++bb) {
-                    const uint64_t o = cc + 4 * dd + bb;
-                    const uint32_t byte = rows[var * stride + (o < bpv ? o : bpv - 1)];
-                    x |= (o < bpv ? byte : 0u) << (8 * bb);
-                }
-                v[dd] = x;
-            }
-        }
-        rb[d] = v;
-        rw[d] = plan.flipmask[rc >> 5];  // used only at store time: no wait here
-        rf[d] = (uint32_t)(rc & 31) | (ok ? 32u : 0u);
-        const u32x4_t *bs = bfrag + (st / kKStep) * tiles_total * 64;
-#pragma unroll
-        for (int m = 0; m < kNB; ++m) rB[d][m] = bs[boff[m]];
-    };
-    auto store = [&](int buf, int d) __attribute__((always_inline)) {
-        u32x4_t v = rb[d];
-        const uint32_t f = rf[d];
-        // zero rows outside the split (they add nothing); flip A1 rows:
-        // code c -> c ^ u with u = 11 where both bits are equal (00 <-> 11)
-        const uint32_t keep = (f & 32u) ? 0xFFFFFFFFu : 0u;
-        const uint32_t fl = ((rw[d] >> (f & 31u)) & 1u) ? keep : 0u;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const uint32_t x = v[e] & keep;
-            const uint32_t u = ~(x ^ (x >> 1)) & 0x55555555u;
-            v[e] = x ^ ((u | (u << 1)) & fl);
-        }
-        uint32_t *dst = &tile[buf][(ld_row * kPitch + ld_chunk * 16) / 4];
-        dst[0] = v[0]; dst[1] = v[1]; dst[2] = v[2]; dst[3] = v[3];
-#pragma unroll
-        for (int m = 0; m < kNB; ++m) btile[buf][tid + m * 64 * WAVES] = rB[d][m];
-    };
-
-    f32x4_t acc[4][TMAX];
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int j = 0; j < TMAX; ++j) acc[t][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-
-    // Flush one scale block: FP64 for weight columns, u32 for counts.  Each
-    // tile is staged through LDS so the read-modify-write of the partials is
-    // one short loop over 128-byte rows.  `empty` writes zeros (a split
-    // without rows must still leave zeros in its partial slots).
-    auto flush = [&](uint64_t blk, bool first, bool empty) __attribute__((always_inline)) {
-        // compile-time tile index (fold below): acc[t][j] must stay in registers
-        auto flush_tile = [&](auto jc) __attribute__((always_inline)) {
-            constexpr int j = decltype(jc)::value;
-            if ((uint32_t)j >= ntile) return;
-            lds_barrier();
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) stage[wave][(16 * lg + 4 * q + t) * 16 + li] = acc[t][j][q];
-                acc[t][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-            }
-            lds_barrier();
-            const uint32_t tl = tile_begin + j;
-            const bool wtile = tl < wt;
-            const uint32_t c = wtile ? tl * 16 + li : (tl - wt) * 16 + li;
-            const uint32_t ncols = wtile ? plan.ncol : plan.nclass;
-            if (c < ncols) {
-                const double sc = (wtile && !empty) ? plan.scales[blk * plan.ncol + c] : 0.0;
-#pragma unroll 1
-                for (int i = 0; i < 16; ++i) {
-                    const uint32_t pl = 4 * i + lg;  // person of the wave, 0..63
-                    const uint64_t p = pw0 + pl;
-                    if (p >= num_people) continue;
-                    const float v = stage[wave][pl * 16 + li];
-                    const uint64_t row = (uint64_t)s * num_people + p;
-                    if (wtile) {
-                        double *dd = part + row * plan.ncol + c;
-                        const double x = (double)v * sc;
-                        *dd = first ? x : *dd + x;
-                    } else {
-                        uint32_t *dd = cpart + row * plan.nclass + c;
-                        const uint32_t x = (uint32_t)v;
-                        *dd = first ? x : *dd + x;
-                    }
-                }
-            }
-        };
-        [&]<int... J>(std::integer_sequence<int, J...>) __attribute__((always_inline)) {
-            (flush_tile(std::integral_constant<int, J>{}), ...);
-        }(std::make_integer_sequence<int, TMAX>{});
-    };
-
-    // Per k-step: eight LDS bytes per lane -> selectors for the four M-tiles
-    // -> dose (and missing) fragments; every B fragment is read from LDS once
-    // and feeds the four M-tiles.
-    auto compute = [&](int buf) __attribute__((always_inline)) {
-        const uint8_t *tb = (const uint8_t *)tile[buf];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            uint32_t x[4][4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t b0 = tb[(32 * h + 8 * lg + 2 * q) * kPitch + wave * 16 + li];
-                const uint32_t b1 = tb[(32 * h + 8 * lg + 2 * q + 1) * kPitch + wave * 16 + li];
-                const uint32_t pp = (b0 | (b1 << 16)) * 0x0101u;
-#pragma unroll
-                for (int t = 0; t < 4; ++t) x[q][t] = sel_pair(pp >> (2 * t));
-            }
-            Frag A[4];
-#pragma unroll
-            for (int t = 0; t < 4; ++t)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) A[t].u[q] = dose_pair(x[q][t]);
-#pragma unroll
-            for (int j = 0; j < TMAX; ++j) {
-                if ((uint32_t)j < nw) {
-                    Frag B;
-                    const u32x4_t bv = btile[buf][(h * TMAX + j) * 64 + lane];
-                    B.u[0] = bv[0]; B.u[1] = bv[1]; B.u[2] = bv[2]; B.u[3] = bv[3];
-#pragma unroll
-                    for (int t = 0; t < 4; ++t)
-                        acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[t].v, B.v, acc[t][j], 0, 0, 0);
-                }
-            }
-            if (need_miss) {
-                Frag M[4];
-#pragma unroll
-                for (int t = 0; t < 4; ++t)
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) M[t].u[q] = miss_pair(x[q][t]);
-#pragma unroll
-                for (int j = 0; j < TMAX; ++j) {
-                    if ((uint32_t)j >= nw && (uint32_t)j < ntile) {
-                        Frag B;
-                        const u32x4_t bv = btile[buf][(h * TMAX + j) * 64 + lane];
-                        B.u[0] = bv[0]; B.u[1] = bv[1]; B.u[2] = bv[2]; B.u[3] = bv[3];
-#pragma unroll
-                        for (int t = 0; t < 4; ++t)
-                            acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(M[t].v, B.v, acc[t][j], 0, 0, 0);
-                    }
-                }
-            }
-        }
-    };
-
-    if (r_begin >= r_end) {
-        flush(0, true, true);
-        return;
-    }
-    // Stages are processed in groups of D (one ring turn); the first stage is
-    // aligned down to a group boundary so scale-block boundaries (every
-    // kFlushRows rows, a multiple of D*kStage) always fall between groups and
-    // the flush stays out of the unrolled steps.
-    static_assert(kFlushRows % (D * kStage) == 0, "scale blocks must hold whole ring turns");
-    const uint64_t st0 = (r_begin / (D * kStage)) * (D * kStage);
-    const uint64_t nst = (r_end - st0 + kStage - 1) / kStage;
-#pragma unroll
-    for (int d = 0; d < D - 1; ++d) issue(d, st0 + (uint64_t)d * kStage);
-    bool first = true;
-    // One pipeline step per stage; `d` is a compile-time ring slot so the
-    // ring stays in fixed registers (no copies that would force waits).
-    auto step = [&](auto dc, uint64_t i) __attribute__((always_inline)) {
-        constexpr int d = decltype(dc)::value;
-        issue((d + D - 1) % D, st0 + (i + D - 1) * kStage);
-        const int buf = (int)(i & 1);
-        store(buf, d);
-        lds_barrier();
-        compute(buf);
-    };
-    for (uint64_t i0 = 0; i0 < nst; i0 += D) {
-        [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
-            ((i0 + I < nst ? (step(std::integral_constant<int, I>{}, i0 + I), 0) : 0), ...);
-        }(std::make_integer_sequence<int, D>{});
-        const uint64_t st_next = st0 + (i0 + D) * kStage;
-        if ((st_next % kFlushRows) == 0 || i0 + D >= nst) {
-            flush((st0 + i0 * kStage) / kFlushRows, first, false);
-            first = false;
-        }
-    }
-}
-
 static uint32_t splits_for(uint64_t wgs_per_split) {
     // enough workgroups to fill 256 CUs several times; a multiple of 8 so
     // every XCD owns whole splits
@@ -825,56 +496,10 @@ static uint32_t splits_for(uint64_t wgs_per_split) {
     return (uint32_t)(s < 8 ? 8 : (s > 64 ? 64 : s));
 }
 
-// 8 waves: each workgroup reads 128 contiguous bytes of a row (whole lines)
-static int mfma_waves(uint32_t) { return 8; }
-
-uint32_t mfma_splits(uint64_t num_people, uint32_t tiles) {
-    const uint32_t nt = tiles < (uint32_t)kMfmaMaxTiles ? tiles : (uint32_t)kMfmaMaxTiles;
-    const uint64_t per = 64ull * mfma_waves(nt ? nt : 1);
-    return splits_for((num_people + per - 1) / per);
-}
-
 uint32_t fp32_splits(uint64_t num_people, uint64_t num_scores) {
     const uint64_t pt = ((num_people + 3) / 4 + 255) / 256;
     const uint64_t chunks = (num_scores + kFp32ScoreChunk - 1) / kFp32ScoreChunk;
     return splits_for(pt * chunks);
-}
-
-hipError_t launch_score_mfma(const DevicePlan &plan, const uint8_t *rows, uint64_t stride,
-                             uint64_t bpv, uint64_t v0, uint64_t p0, uint64_t p1,
-                             uint64_t num_people, uint32_t splits,
-                             uint32_t tile_begin, uint32_t tile_end, bool identity, double *part,
-                             uint32_t *cpart, hipStream_t stream) {
-    const bool vec = ((reinterpret_cast<uintptr_t>(rows) | stride) & 15u) == 0;
-    // splits start on 64-row stage boundaries; scale blocks may straddle them
-    const uint64_t split_base = (p0 / 64) * 64;
-    uint64_t rows_per_split = (p1 - split_base + splits - 1) / splits;
-    rows_per_split = (rows_per_split + 63) / 64 * 64;
-    for (uint32_t tb = tile_begin; tb < tile_end; tb += kMfmaMaxTiles) {
-        const uint32_t nt = (tile_end - tb) < (uint32_t)kMfmaMaxTiles ? (tile_end - tb) : (uint32_t)kMfmaMaxTiles;
-#define GNM_LAUNCH_MFMA(TM, WV, DD, VC, ID)                                                                  \
-    do {                                                                                                     \
-        const uint32_t pt = (uint32_t)((num_people + 64 * WV - 1) / (64 * WV));                              \
-        const uint64_t wgs = (uint64_t)pt * splits; /* splits is a multiple of 8 */                          \
-        hipLaunchKernelGGL((k_score_mfma<TM, WV, DD, VC, ID>), dim3((unsigned)wgs), dim3(64 * WV), 0, stream,    \
-                           plan, rows, stride, bpv, v0, p0, p1, split_base, rows_per_split, splits, pt,      \
-                           num_people, tb, nt, part, cpart);                                                 \
-    } while (0)
-#define GNM_LAUNCH_MFMA_V(TM, WV, DD)                                                                        \
-    do {                                                                                                     \
-        if (vec) {                                                                                           \
-            if (identity) GNM_LAUNCH_MFMA(TM, WV, DD, true, true); else GNM_LAUNCH_MFMA(TM, WV, DD, true, false); \
-        } else {                                                                                             \
-            if (identity) GNM_LAUNCH_MFMA(TM, WV, DD, false, true); else GNM_LAUNCH_MFMA(TM, WV, DD, false, false); \
-        }                                                                                                    \
-    } while (0)
-        GNM_LAUNCH_MFMA_V(kMfmaMaxTiles, 8, 4);
-#undef GNM_LAUNCH_MFMA_V
-#undef GNM_LAUNCH_MFMA
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
 }
 
 // ---------------------------------------------------------------------------

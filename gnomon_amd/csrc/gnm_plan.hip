@@ -13,7 +13,7 @@
 //   k_plan_exps     per (scale block, score) the power-of-two scale 2^e with
 //                   max |w| <= qmax * 2^e, and the limb scales;
 //   k_plan_frag_i8  split-int8 B fragments, [k-step][tile][lane][16 i8];
-//   k_plan_frag_bf16 split-BF16 B fragments, [k-step][tile][lane][8 bf16];
+//   k_plan_frag_bf16 split-BF16 B fragments, [stage][tile][k-step][lane][8 bf16];
 //   k_plan_w64      FP32-path weights (W rounded once to f32, held as f64).
 #include "gnm_internal.hpp"
 
@@ -107,23 +107,24 @@ __global__ __launch_bounds__(256) void k_plan_frag_i8(const double *__restrict__
     frag[idx] = make_uint4(word[0], word[1], word[2], word[3]);
 }
 
-// one thread per (k-step, tile, lane): 8 bf16, rows 32ks + 8g + j of column
-// 16*tile + n
+// one thread per (64-row stage, tile, k-step h, lane): 8 bf16, rows
+// 64 st + 16 g + 8 h + j (j = 0..7) of column 16*tile + n -- the row order in
+// which the BF16 scoring kernel decodes a lane's 16 rows into two k-steps
 __global__ __launch_bounds__(256) void k_plan_frag_bf16(const double *__restrict__ wrow, uint64_t R, uint64_t K,
                                                         const int *__restrict__ uexp, uint32_t L, uint32_t ncol,
-                                                        uint32_t wtiles, uint32_t tiles, uint64_t nks,
+                                                        uint32_t wtiles, uint32_t tiles, uint64_t nst,
                                                         const uint8_t *__restrict__ count_row,
                                                         const uint32_t *__restrict__ rowcls, uint32_t cw,
                                                         uint32_t nclass, uint4 *__restrict__ frag) {
     const uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= nks * tiles * 64) return;
-    const uint32_t lane = (uint32_t)(idx % 64), tile = (uint32_t)((idx / 64) % tiles);
-    const uint64_t ks = idx / (64ull * tiles);
+    if (idx >= nst * tiles * 128) return;
+    const uint32_t lane = (uint32_t)(idx % 64), h = (uint32_t)((idx / 64) % 2), tile = (uint32_t)((idx / 128) % tiles);
+    const uint64_t st = idx / (128ull * tiles);
     const uint32_t n = lane % 16, g = lane / 16;
     const uint32_t col = tile * 16 + n;
     uint32_t word[4] = {0u, 0u, 0u, 0u};
     for (uint32_t j = 0; j < 8; ++j) {
-        const uint64_t r = ks * kKStep + 8 * g + j;
+        const uint64_t r = st * 64 + 16 * g + 8 * h + j;
         if (r >= R) break;
         uint32_t val = 0;
         if (tile < wtiles) {
@@ -131,7 +132,7 @@ __global__ __launch_bounds__(256) void k_plan_frag_bf16(const double *__restrict
                 const uint32_t limb = col / (uint32_t)K;
                 const uint64_t k = col % K;
                 const double w = wrow[r * K + k];
-                const uint64_t blk = ks * kKStep / kFlushRows;
+                const uint64_t blk = r / kBfFlushRows;
                 const int e = uexp[blk * K + k];
                 if (w != 0.0 && e != kBadExp) {
                     int32_t dig[4];
@@ -180,7 +181,9 @@ hipError_t launch_plan_frag(bool int8, const double *wrow, uint64_t R, uint64_t 
                             uint32_t L, uint32_t ncol, uint32_t wtiles, uint32_t tiles, uint64_t nks,
                             const uint8_t *count_row, const uint32_t *rowcls, uint32_t cw, uint32_t nclass,
                             void *frag, hipStream_t stream) {
-    const uint64_t n = nks * tiles * 64;
+    // 16 bytes per (int8 k-step of 64 rows, tile, lane), or per (64-row
+    // stage, tile, BF16 k-step, lane) with nks = stages
+    const uint64_t n = nks * tiles * (int8 ? 64 : 128);
     if (n == 0) return hipSuccess;
     if (int8)
         hipLaunchKernelGGL(k_plan_frag_i8, dim3(blocks_for(n)), dim3(256), 0, stream, wrow, R, K, uexp, fr, L, ncol,

@@ -23,7 +23,7 @@ from tests.helpers import gpu_tests_genotype, normwise_rel, pack_bed, reference_
 pytestmark = pytest.mark.gpu
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))
 LD = os.path.join(os.path.dirname(__file__), "golden", "ld.bed")
-TOL = {"fp32": None, "bf16x3": 1e-6, "bf16x2": 1e-4, "i8x4": 1e-6, "i8x3": 1e-5}
+TOL = {"fp32": None, "bf16x4": 1e-6, "bf16x3": 1e-6, "bf16x2": 1e-4, "i8x4": 1e-6, "i8x3": 1e-5}
 
 
 def _flush(msg):
@@ -41,7 +41,7 @@ def _check_scores(S, C, S64, C64, precision):
         return
     ok = reference_agree(S, S64, 1e-6)
     assert ok.all(), f"{(~ok).sum()} cells outside the 1e-6 rule; worst {np.abs(S - S64).max()}"
-    if precision in ("bf16x3", "i8x4"):
+    if precision in ("bf16x4", "bf16x3", "i8x4"):
         assert normwise_rel(S, S64).max() <= TOL[precision]
 
 
@@ -114,7 +114,7 @@ def test_unpack_ld_fixture(orc):
     assert np.array_equal(d_out.cpu().numpy().reshape(V, N), orc.decode_rows(rows, N))
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16x2", "i8x4", "i8x3"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x4", "bf16x3", "bf16x2", "i8x4", "i8x3"])
 def test_plain_small_config(orc, precision):
     """2,000 people x 50,000 variants x 1 score, HWE p~U(0.05,0.5), 1% missing,
     w ~ N(0, 0.01^2), effect allele A1 for a random half."""
@@ -199,7 +199,7 @@ def test_reference_golden_vectors_on_device(precision):
     assert abs(S[0, 0] - g["sum"]) < 1e-6 and C[0, 0] == 0
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "i8x4"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x4", "bf16x3", "i8x4"])
 def test_split_self_test_known_answer(precision):
     """Inputs of the reference's split self-test (cuda_backend.rs:2432-2438):
     dosage[p][v] = (64p+v) % 3, weight[v][k] = ((7(64v+k)) mod 101 - 50)/13.
@@ -222,7 +222,7 @@ def test_split_self_test_known_answer(precision):
         # max|w| <= qmax u, so u < 2 max|w| / qmax and |w - Q u| <= u/2 <
         # max|w| / qmax per weight (qmax = 2^24 - 2^16 for three BF16 limbs,
         # 127 * 256^3 for four int8 limbs); |dS| <= that times sum_v d
-        qmax = {"bf16x3": 2.0**24 - 2.0**16, "i8x4": 127.0 * 256.0**3}[precision]
+        qmax = {"bf16x4": 2.0**32 - 2.0**24, "bf16x3": 2.0**24 - 2.0**16, "i8x4": 127.0 * 256.0**3}[precision]
         bound = np.abs(W64).max() / qmax * dos.sum(1, keepdims=True)
         assert np.all(np.abs(S - exact_f32w) <= bound + 1e-12)
         assert normwise_rel(S, exact_f32w).max() <= 1e-6
@@ -380,7 +380,69 @@ def test_int8_two_set_shape_many_scores(orc):
     assert np.array_equal(out[18][0], out[0][0])
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "i8x4"])
+@pytest.mark.parametrize("precision", ["bf16x4", "bf16x3"])
+def test_bf16_scale_blocks_and_launch_shapes(orc, precision):
+    """100,000 variants = three full split-BF16 scale blocks of 32,768 rows
+    and a partial one; 700 people (ragged); 24 scores with per-cell presence
+    (6 or 5 weight tiles and 2 count tiles: mixed, count-only and partly
+    unused groups depending on the shape); rows in three blocks that start
+    inside scale blocks.  Every tile-group size (3, 5, 6, 8, 9) gives
+    bit-identical results (exact FP32 block sums, the same FP64 order), and
+    they match the FP64 oracle."""
+    _flush(f"bf16 blocks {precision}")
+    rng = np.random.default_rng(33)
+    N, V, K = 700, 100_000, 24
+    rows = orc.synth_rows(8, N, 3, V, int(0.02 * 2**24), int(0.5 * 2**24), 1311)
+    W = rng.normal(0, 0.01, (V, K)).astype(np.float32)
+    W[40_000:50_000] *= 1e-3            # part of one block with much smaller weights
+    present = rng.random((V, K)) < 0.9
+    present[:, :4] = True
+    csr = csr_from_dense(W, rng.random((V, K)) < 0.5, present)
+    S64, C64 = orc.score_fp64(rows, csr, N)
+    first = None
+    for tp in (0, 3, 5, 6, 8, 9):
+        _flush(f"  tiles_per_pass={tp}")
+        with ScoringContext(csr, N, precision=precision, force=True) as ctx:
+            ctx.set_launch_config(tp, 0)
+            for a, b in [(0, 30_000), (30_000, 70_001), (70_001, V)]:
+                ctx.score_rows(rows[a:b], a)
+            S, C = ctx.finish()
+        if first is None:
+            first = (S, C)
+            if precision == "bf16x4":
+                _check_scores(S, C, S64, C64, precision)
+            else:
+                # 24 bits of a 32768-row block maximum: the block with 1000x
+                # smaller weights keeps ~14 bits of them, so only the
+                # normwise bound of the representation holds
+                assert np.array_equal(C, C64)
+                assert normwise_rel(S, S64).max() <= 1e-6
+        else:
+            assert np.array_equal(C, first[1])
+            assert np.array_equal(S, first[0]), f"shape {tp} differs"
+
+
+def test_bf16_launch_config_rejects_int8_shapes(orc):
+    """The split-BF16 path takes tile groups of 3, 5, 6, 8 or 9 and two-stage
+    batches; other shapes are refused and the context still scores."""
+    from gnomon_amd._lib import GnmError
+    rng = np.random.default_rng(43)
+    N, V, K = 300, 2_000, 3
+    rows = orc.synth_rows(8, N, 3, V, int(0.02 * 2**24), int(0.5 * 2**24), 1409)
+    csr = csr_from_dense(rng.normal(0, 0.01, (V, K)).astype(np.float32), rng.random((V, K)) < 0.5,
+                         np.ones((V, K), bool))
+    S64, C64 = orc.score_fp64(rows, csr, N)
+    with ScoringContext(csr, N, precision="bf16x4", force=True) as ctx:
+        for tp, bs in [(4, 0), (7, 0), (18, 2), (3, 4), (2, 0)]:
+            with pytest.raises(GnmError):
+                ctx.set_launch_config(tp, bs)
+        ctx.set_launch_config(5, 2)
+        ctx.score_rows(rows, 0)
+        S, C = ctx.finish()
+    _check_scores(S, C, S64, C64, "bf16x4")
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16x4", "i8x4"])
 @pytest.mark.parametrize("N,V,K", [(1, 65, 1), (3, 7, 2), (5, 130, 3), (33, 1, 2), (63, 200, 5)])
 def test_tiny_and_ragged_shapes(orc, precision, N, V, K):
     """Single person, fewer than 4 people (one partial byte), one variant,
