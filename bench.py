@@ -1,25 +1,28 @@
 """Benchmark of the scoring hot path on one MI355X.
 
-Default workload: plain FP32-weight scoring at biobank scale -- 100,000 people
-x 1,000,000 variants x 8 scores, HWE genotypes (p ~ U(0.01, 0.5), 2%
-missing), weights ~ N(0, 0.01^2), effect allele A1 for a random half.  The
-25 GB .bed is generated on the device (counter-based hash) and stays resident
-in HBM; one step is one pass of the scoring kernels over every variant,
-accumulated into the FP64 result.
+Default workload (the precision configuration): 100,000 people x 1,000,000
+variants x 64 scores, HWE genotypes (A2 frequency ~ U(0.01, 0.5), 2%
+missing), weights ~ Laplace(0, 1e-3) with 30% exact zeros kept present, the
+effect allele A1 for a random half of the variants.  The 25 GB .bed is
+generated on the device (counter-based hash) and stays resident in HBM; one
+step is one pass of the scoring kernels over every variant, accumulated into
+the FP64 result.
 
-Headline path: split-int8 MFMA with four limbs (i8x4: weights as 4 signed
-8-bit limbs per power-of-two block scale, exact int32 block sums, FP64 across
-blocks); the plain FP32 path and the split-BF16 path (bf16x3) are timed on the
-same rows for comparison.
+Headline path: split-BF16 MFMA with four BF16 limbs (bf16x4: weights as
+integer-valued BF16 digits per power-of-two block scale, exact FP32 block
+sums, FP64 across blocks).  The plain FP32 path and the split-int8 path
+(i8x4) are timed on the same rows for comparison.
 
 Prints one JSON line: metric svs/s = people * variants * scores / second,
-plus the roofline of the dominant kernel (HIP-event time on the library's
-compute stream) and the CPU baseline (restated reference CPU engine on a
-bounded sample of the same workload, all host cores).
+the roofline of the dominant kernel (MFMA-bound for the split paths: executed
+matrix operations over HIP-event time on the library's compute stream), the
+per-score error of each path against the FP64 oracle, the standalone unpack
+stage against the HBM roofline, and the CPU baseline (restated reference CPU
+engine on a bounded sample of the same workload, with the host's CPU facts).
 
-Other modes: --workload precision_64 (per-score error report of every path
-against the FP64 oracle), --workload adjoint_16 (differentiable path: 20
-gradient steps, finite-difference and FP64 gradient checks),
+Other modes: --workload plain_fp32_biobank (8 scores), --workload adjoint_16
+(differentiable path: 20 gradient steps, finite-difference and FP64 gradient
+checks), --workload batching_200 (200 score files in one grouped launch),
 --calibrate-copy MIB (a plain copy of known size for the HBM counters).
 """
 from __future__ import annotations
@@ -38,7 +41,44 @@ HBM_PEAK_GBS = 8000.0
 DTYPE = {"fp32": "f32*{0,1,2}->f64", "bf16x2": "bf16x2->f32->f64", "bf16x3": "bf16x3->f32->f64",
          "bf16x4": "bf16x4->f32->f64",
          "i8x3": "i8x3->i32->f64", "i8x4": "i8x4->i32->f64"}
-MFMA_BF16_PEAK_TFLOPS = 2500.0
+# published dense MFMA peaks of one MI355X (no sparsity)
+MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "int8": 5000.0}
+
+
+def cpu_facts():
+    """Host CPU model, logical CPU count, the CPUs this process may run on,
+    and whether simultaneous multithreading is on (Linux sysfs/procfs)."""
+    facts = {"nproc": os.cpu_count()}
+    try:
+        facts["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                facts["model"] = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        facts["smt_active"] = open("/sys/devices/system/cpu/smt/active").read().strip() == "1"
+    except OSError:
+        pass
+    return facts
+
+
+def mfma_work(precision, N, V, K, nclass=1):
+    """Executed matrix operations of one scoring pass of a split path: the
+    kernel multiplies every (person, program row) dosage by every column of
+    its 16-column tiles, 2 operations (multiply, add) each.  Program rows = V
+    here (flips are per variant, so no variant needs two rows); tiles =
+    ceil(limbs * K / 16) weight tiles, plus ceil(nclass / 16) count tiles
+    when scores do not share one membership class (one class is counted on
+    the vector ALUs)."""
+    limbs = int(precision[-1])
+    tiles = (limbs * K + 15) // 16 + (0 if nclass == 1 else (nclass + 15) // 16)
+    kind = "int8" if precision.startswith("i8") else "bf16"
+    return 2 * N * V * tiles * 16, kind, tiles
 
 
 def log(msg):
@@ -57,7 +97,8 @@ def profiled_traffic(kernel, workload, precision, alg_bytes, live_ms):
 
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         v = read_values(path)
-        if (v.get("kernel"), v.get("workload"), v.get("precision", "")) != (kernel, workload, precision):
+        if not (v.get("kernel", "").startswith(kernel) and (v.get("workload"), v.get("precision", "")) ==
+                (workload, precision)):
             continue
         if "traffic_bytes_per_launch" not in v or int(float(v.get("algorithmic_bytes_per_launch", -1))) != alg_bytes:
             continue
@@ -204,39 +245,38 @@ def measure_host_rows(d_bed, csr, N, V, K, stride, args):
     return out
 
 
-def run_precision_report(w, csr, d_bed, stride, args):
-    """Per-score error of each precision at full size: against the CPU FP64
-    oracle on the first 128 people (all variants), and against the plain FP32
-    path (exact products, FP64 accumulation) on all people."""
+def run_precision_report(w, csr, d_bed, stride, paths, sample=256):
+    """Per-score error of each path at full size against the CPU FP64 oracle
+    on the first `sample` people (all variants, all scores; the oracle runs
+    over slices of people on the host threads): (i) the largest relative
+    error over people with |S64| >= 1e-3 max|S64|, (ii) normwise
+    max|S - S64| / max|S64|, (iii) the reference's AVG agreement rule; counts
+    compared exactly."""
     import numpy as np
 
     from gnomon_amd.pipeline import ScoringContext
     from oracle import oracle as orc
 
     N, V, K = w.num_people, w.num_variants, w.num_scores
-    sample = 128
     t0 = time.time()
-    rows = d_bed.view(V, stride)[:, :sample // 4].cpu().numpy()
-    S64, C64 = orc.score_fp64(rows, csr, sample)
+    rows = d_bed.view(V, stride)[:, :(sample + 3) // 4].cpu().numpy()
+    S64, C64 = orc.score_fp64_parallel(rows, csr, sample)
     log(f"FP64 oracle on {sample} people x {V} variants x {K} scores ({time.time() - t0:.1f}s)")
-    out, full = {}, {}
-    for prec in ("fp32", "bf16x3", "bf16x2", "i8x4", "i8x3"):
+    out = {}
+    for prec in paths:
         with ScoringContext(csr, N, precision=prec, force=True) as c:
             c.score_rows_device(d_bed.data_ptr(), stride, 0, V)
             S, C = c.finish()
-        full[prec] = S
         rel, nw = _errors(S[:sample], S64)
-        out[prec] = {"vs_fp64_oracle_sample_people": sample,
+        out[prec] = {"max_rel_err_per_sample": float(rel.max()), "max_normwise_err": float(nw.max()),
                      "max_rel_err_per_score": [float(f"{x:.3e}") for x in rel],
-                     "max_rel_err": float(rel.max()), "max_normwise_err": float(nw.max()),
                      "counts_exact": bool(np.array_equal(C[:sample], C64)),
                      "reference_avg_rule_pass_fraction": _avg_rule_fraction(S[:sample], C[:sample], S64, C64,
                                                                              csr.score_variant_counts)}
-        log(f"[{prec}] vs FP64 oracle: max rel {rel.max():.3e}, normwise {nw.max():.3e}")
-    for prec in ("bf16x3", "bf16x2", "i8x4", "i8x3"):
-        rel, nw = _errors(full[prec], full["fp32"])
-        out[prec]["vs_fp32_path_all_people"] = {"max_rel_err": float(rel.max()), "max_normwise_err": float(nw.max())}
-    return out
+        log(f"[{prec}] vs FP64 oracle: per-sample max rel {rel.max():.3e}, normwise {nw.max():.3e}")
+    return {"oracle": "FP64 (compensated sums), first %d people, all variants and scores" % sample,
+            "metric_i": "max over people with |S64| >= 1e-3 max|S64| of |S - S64| / |S64|, per score",
+            "paths": out}
 
 
 def run_differentiation(w, args):
@@ -461,22 +501,57 @@ def run_batching(w, args):
     print(json.dumps(out), flush=True)
 
 
+def cpu_baseline(d_bed, csr, N, V, K, bpv, stride, target_s=15.0):
+    """The restated reference CPU engine on a bounded sample of the same
+    workload: a short first pass estimates its rate, then a sample sized for
+    about target_s seconds is timed.  Threads: the CPUs this process may run
+    on, at most 16 (the host share of one GPU on the measurement boxes)."""
+    from gnomon_amd.prep import ScoreCsr
+    from oracle import oracle as orc
+
+    facts = cpu_facts()
+    threads = min(facts.get("affinity_cpus", facts["nproc"] or 1), 16)
+
+    def head(vs):
+        off = int(csr.row_offsets[vs])
+        return ScoreCsr(csr.weights[:off], csr.corrections[:off], csr.columns[:off],
+                        csr.row_offsets[:vs + 1], K, csr.baseline, csr.score_variant_counts)
+
+    vs = min(V, 2000)
+    rows = d_bed.view(V, stride)[:vs, :bpv].cpu().numpy()
+    t1 = time.perf_counter()
+    orc.refcpu_run(rows, head(vs), N, threads=threads)
+    rate = N * vs * K / (time.perf_counter() - t1)
+    vs = int(min(V, max(2000, rate * target_s / (N * K))))
+    rows = d_bed.view(V, stride)[:vs, :bpv].cpu().numpy()
+    t1 = time.perf_counter()
+    _, _, used = orc.refcpu_run(rows, head(vs), N, threads=threads)
+    dt = time.perf_counter() - t1
+    out = {"value": N * vs * K / dt, "unit": "sample*variant*score/s", "cores": used, "kind": "port",
+           "sample": f"restated reference CPU engine (oracle/gnomon_oracle.c) on {N} people x {vs} variants x "
+                     f"{K} scores of the same workload, {dt:.1f} s",
+           "host": facts}
+    log(f"cpu baseline: {out['value']:.3e} svs/s on {used} threads ({facts})")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="plain_fp32_biobank")
-    ap.add_argument("--precision", default="i8x4", choices=["fp32", "bf16x2", "bf16x3", "bf16x4", "i8x3", "i8x4"])
-    ap.add_argument("--compare", default="fp32,bf16x3",
+    ap.add_argument("--workload", default="precision_64")
+    ap.add_argument("--precision", default="bf16x4",
+                    choices=["fp32", "bf16x2", "bf16x3", "bf16x4", "i8x3", "i8x4"])
+    ap.add_argument("--compare", default="fp32,i8x4",
                     help="comma-separated further paths timed on the same rows for the report ('' = none)")
     ap.add_argument("--variants", type=int, default=0, help="override the variant count")
-    ap.add_argument("--cpu-sample-variants", type=int, default=300000,
-                    help="variants of the CPU-baseline sample (about 10 s on 16 host threads at 100k people)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="check a slice against the FP64 oracle")
     ap.add_argument("--no-unpack", action="store_true", help="skip the standalone unpack-stage measurement")
-    ap.add_argument("--no-report", action="store_true", help="skip the per-score precision report (precision_64)")
+    ap.add_argument("--no-report", action="store_true", help="skip the per-score precision report")
     ap.add_argument("--host-rows", action="store_true",
                     help="also time a pass with the rows in host memory (PCIe-inclusive, not the headline value)")
     ap.add_argument("--launch", default="", metavar="TP,BS",
@@ -553,18 +628,17 @@ def main():
     check = None
     if args.check:
         from oracle import oracle as orc
-        S, C, done = ctx.read_accumulators()
+        from gnomon_amd.prep import ScoreCsr
         nrow = min(V, 2000)
         rows = d_bed.view(V, stride)[:nrow, :bpv].cpu().numpy()
         # a fresh single-pass context over the first rows vs the FP64 oracle
-        from gnomon_amd.prep import ScoreCsr
         off = int(csr.row_offsets[nrow])
         head = ScoreCsr(csr.weights[:off], csr.corrections[:off], csr.columns[:off],
                         csr.row_offsets[:nrow + 1], K, csr.baseline, csr.score_variant_counts)
         with ScoringContext(head, N, precision=args.precision, force=True) as c2:
             c2.score_rows(rows, 0)
             S2, C2 = c2.finish()
-        S64, C64 = orc.score_fp64(rows, head, N)
+        S64, C64 = orc.score_fp64_parallel(rows, head, N)
         rel = (np.abs(S2 - S64).max(0) / np.abs(S64).max(0)).max()
         check = {"variants": nrow, "counts_exact": bool(np.array_equal(C2, C64)), "max_normwise_rel": float(rel)}
         log(f"check: {check}")
@@ -575,45 +649,47 @@ def main():
         c2, ms2, main2 = run_path(prec)
         c2.close()
         compare.append({"precision": prec, "ms_per_step": ms2, "value": svs / (ms2 / 1e3),
-                        "main_kernel_ms": main2, "speedup_of_headline": ms2 / ms_step})
+                        "main_kernel_ms": main2, "headline_speedup": ms2 / ms_step})
 
-    # roofline of the dominant kernel: algorithmic bytes = packed .bed +
-    # one f32 weight matrix + f64 scores and u32 counts
+    # algorithmic HBM bytes of one pass: packed .bed + one f32 weight matrix
+    # + f64 scores and u32 counts
     alg_bytes = V * bpv + V * K * 4 + N * K * (8 + 4)
-    achieved = alg_bytes / (main_ms / 1e3) / 1e9
-    kname = {"fp32": "k_score_fp32", "i8x3": "k_score_i8", "i8x4": "k_score_i8"}.get(args.precision, "k_score_mfma")
+    hbm_gbs = alg_bytes / (main_ms / 1e3) / 1e9
+    kname = {"fp32": "k_score_fp32"}.get(args.precision, "k_score_mx")
     traffic, tsrc = profiled_traffic(kname, w.name, args.precision, alg_bytes, main_ms)
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc,
-                "kernel": kname, "kernel_ms": round(main_ms, 4), "algorithmic_bytes": alg_bytes}
+    if args.precision == "fp32":
+        roofline = {"bound": "hbm", "achieved": round(hbm_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(hbm_gbs / HBM_PEAK_GBS, 4)}
+    else:
+        ops, kind, tiles = mfma_work(args.precision, N, V, K)
+        tflops = ops / (main_ms / 1e3) / 1e12
+        peak = MFMA_PEAK_TFLOPS[kind]
+        roofline = {"bound": "mfma", "achieved": round(tflops, 1), "peak": peak,
+                    "unit": "TFLOP/s" if kind == "bf16" else "TOP/s", "frac": round(tflops / peak, 4),
+                    "mfma": kind, "executed_ops": ops, "tiles": tiles,
+                    "hbm_achieved_gbs": round(hbm_gbs, 1), "hbm_frac": round(hbm_gbs / HBM_PEAK_GBS, 4)}
+    roofline.update({"traffic": traffic, "traffic_source": tsrc, "kernel": kname, "kernel_ms": round(main_ms, 4),
+                     "algorithmic_bytes": alg_bytes})
 
     precision_report = None
-    if w.name == "precision_64" and not args.no_report:
-        precision_report = run_precision_report(w, csr, d_bed, stride, args)
-    unpack = None if args.no_unpack or w.name != "plain_fp32_biobank" else measure_unpack(d_bed, N, V, bpv, stride, max(3, min(args.steps, 10)), w.name)
+    if not args.no_report:
+        paths = [args.precision] + [c["precision"] for c in compare]
+        if w.name == "precision_64":
+            paths += [p for p in ("bf16x3",) if p not in paths]
+        precision_report = run_precision_report(w, csr, d_bed, stride, paths)
+    unpack = None if args.no_unpack else measure_unpack(d_bed, N, V, bpv, stride, max(3, min(args.steps, 10)),
+                                                        w.name)
 
     host_rows = None
     if args.host_rows:
         host_rows = measure_host_rows(d_bed, csr, N, V, K, stride, args)
 
-    cpu = None
-    if not args.no_cpu:
-        from oracle import oracle as orc
-        from gnomon_amd.prep import ScoreCsr
-        vs = min(V, args.cpu_sample_variants)
-        rows = d_bed.view(V, stride)[:vs, :bpv].cpu().numpy()
-        off = int(csr.row_offsets[vs])
-        head = ScoreCsr(csr.weights[:off], csr.corrections[:off], csr.columns[:off],
-                        csr.row_offsets[:vs + 1], K, csr.baseline, csr.score_variant_counts)
-        threads = min(os.cpu_count() or 1, 16)
-        t1 = time.perf_counter()
-        _, _, used = orc.refcpu_run(rows, head, N, threads=threads)
-        dt = time.perf_counter() - t1
-        cpu = {"value": N * vs * K / dt, "unit": "sample*variant*score/s", "cores": used, "kind": "port",
-               "sample": f"restated reference CPU engine on {N} people x {vs} variants x {K} scores of the "
-                         f"same workload ({dt:.2f}s)"}
-        log(f"cpu baseline: {cpu['value']:.3e} svs/s on {used} threads")
+    cpu = None if args.no_cpu else cpu_baseline(d_bed, csr, N, V, K, bpv, stride, args.cpu_seconds)
 
+    fp32 = next((c for c in compare if c["precision"] == "fp32"), None)
+    headline = {"path": args.precision, "vs_fp32_path": round(fp32["ms_per_step"] / ms_step, 3) if fp32 else None}
+    if precision_report:
+        headline["max_rel_err_per_sample"] = precision_report["paths"][args.precision]["max_rel_err_per_sample"]
     out = {
         "metric": "sample*variant*score/s", "value": value, "unit": "sample*variant*score/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
@@ -622,7 +698,7 @@ def main():
         "data": "synthetic (device-generated HWE genotypes, numpy weights)",
         "config": {"workload": w.name, "people": N, "variants": V, "scores": K,
                    "bed_bytes": V * bpv, "precision": args.precision},
-        "roofline": roofline, "cpu_baseline": cpu, "compare": compare, "check": check,
+        "headline": headline, "roofline": roofline, "cpu_baseline": cpu, "compare": compare, "check": check,
         "unpack_stage": unpack, "precision_report": precision_report, "host_rows": host_rows,
     }
     print(json.dumps(out), flush=True)

@@ -95,6 +95,11 @@ def main():
     ap.add_argument("--units", type=float, default=0.0, help="work units per launch, for per-unit counts")
     ap.add_argument("--unit-name", default="unit")
     ap.add_argument("--command", action="append", default=[])
+    ap.add_argument("--mfma-ops", type=float, default=0.0,
+                    help="executed matrix operations per launch (prints achieved TFLOP/s against --mfma-peak)")
+    ap.add_argument("--mfma-peak", type=float, default=2500.0, help="dense MFMA peak, TFLOP/s (or TOP/s)")
+    ap.add_argument("--simds", type=int, default=1024, help="SIMDs of the device (256 CUs x 4)")
+    ap.add_argument("--xcds", type=int, default=8, help="GRBM counters are summed over this many XCDs")
     ap.add_argument("-o", "--output", required=True)
     a = ap.parse_args()
 
@@ -146,6 +151,18 @@ def main():
         if a.units:
             lines += ["", f"{a.unit_name}s per launch: {a.units:.4e}."]
         lines.append("")
+        if "GRBM_GUI_ACTIVE" in cnt:
+            cyc = cnt["GRBM_GUI_ACTIVE"] / a.xcds
+            lines += [f"Clock: {cyc:.4e} GPU cycles per launch (GRBM_GUI_ACTIVE / {a.xcds} XCDs), "
+                      f"{cyc / (avg_ms * 1e-3) / 1e9:.2f} GHz over the average launch.", ""]
+            if "SQ_VALU_MFMA_BUSY_CYCLES" in cnt:
+                busy = cnt["SQ_VALU_MFMA_BUSY_CYCLES"] / (cyc * a.simds)
+                lines += [f"Matrix pipe busy: SQ_VALU_MFMA_BUSY_CYCLES / (cycles x {a.simds} SIMDs) = {busy:.3f}.", ""]
+    if a.mfma_ops:
+        tf = a.mfma_ops / (avg_ms * 1e-3) / 1e12
+        lines += ["## MFMA roofline", "",
+                  f"Executed matrix operations per launch {a.mfma_ops:.4e}; at {avg_ms:.4f} ms that is "
+                  f"{tf:.1f} TFLOP/s, {tf / a.mfma_peak:.3f} of the {a.mfma_peak:.0f} TFLOP/s dense peak.", ""]
     lines += ["## Values read by bench.py", "", "```",
               f"kernel = {a.kernel}", f"workload = {a.workload}", f"precision = {a.precision}",
               f"kernel_avg_ms = {avg_ms:.6f}", f"algorithmic_bytes_per_launch = {a.alg_bytes:.0f}"]

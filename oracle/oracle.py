@@ -102,6 +102,25 @@ def score_fp64(rows, csr, num_people, out_to_fam=None):
     return S, C
 
 
+def score_fp64_parallel(rows, csr, num_people, threads=0):
+    """score_fp64 over slices of people in a thread pool (the C entry point
+    runs without the GIL); identical results to one score_fp64 call."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    threads = threads or min(16, os.cpu_count() or 1)
+    edges = np.linspace(0, num_people, threads + 1).astype(np.int64)
+    parts = [(int(a), int(b)) for a, b in zip(edges[:-1], edges[1:]) if b > a]
+
+    def run(ab):
+        a, b = ab
+        return score_fp64(rows, csr, b - a, np.arange(a, b, dtype=np.uint32))
+
+    with ThreadPoolExecutor(len(parts)) as ex:
+        res = list(ex.map(run, parts))
+    return np.concatenate([r[0] for r in res]), np.concatenate([r[1] for r in res])
+
+
 def refcpu_run(rows, csr, num_people, total_people_in_fam=None, out_to_fam=None,
                threads=0, force_path=-1):
     """Restated reference CPU engine. Returns (scores, counts, threads_used)."""

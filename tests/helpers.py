@@ -57,19 +57,5 @@ def per_sample_rel(S, S64, floor=1e-3):
 
 
 def oracle_fp64_parallel(orc, rows, csr, num_people, threads=0):
-    """The FP64 oracle over slices of people in a thread pool (the oracle's C
-    entry point runs without the GIL); identical to one orc.score_fp64 call."""
-    import os
-    from concurrent.futures import ThreadPoolExecutor
-
-    threads = threads or min(16, os.cpu_count() or 1)
-    edges = np.linspace(0, num_people, threads + 1).astype(np.int64)
-    parts = [(int(a), int(b)) for a, b in zip(edges[:-1], edges[1:]) if b > a]
-
-    def run(ab):
-        a, b = ab
-        return orc.score_fp64(rows, csr, b - a, np.arange(a, b, dtype=np.uint32))
-
-    with ThreadPoolExecutor(len(parts)) as ex:
-        res = list(ex.map(run, parts))
-    return np.concatenate([r[0] for r in res]), np.concatenate([r[1] for r in res])
+    """The FP64 oracle over slices of people in a thread pool."""
+    return orc.score_fp64_parallel(rows, csr, num_people, threads)
