@@ -469,7 +469,7 @@ def run_batching(w, args):
         f"max normwise difference to grouped {worst:.2e} ({time.time() - t2:.1f}s)")
 
     check = None
-    if args.check:
+    if not args.no_report:   # the oracle check of the grouped result runs by default
         from oracle import oracle as orc
         from gnomon_amd.prep import ScoreCsr
         sample = 128
@@ -478,7 +478,7 @@ def run_batching(w, args):
         errs = []
         for k in picks:
             csr_k = csr_from_dense(W[:, k:k + 1], flipped, present[:, k:k + 1])
-            S64, C64 = orc.score_fp64(rows, csr_k, sample)
+            S64, C64 = orc.score_fp64_parallel(rows, csr_k, sample)
             assert np.array_equal(C64[:, 0], C_g[:sample, k])
             errs.append(float(np.abs(S_g[:sample, k] - S64[:, 0]).max() / max(np.abs(S64).max(), 1e-300)))
         check = {"files": picks, "people": sample, "max_normwise_err_vs_fp64": max(errs), "counts_exact": True}

@@ -1,21 +1,24 @@
-"""Autotuner for the launch shape of the split-int8 scoring kernel.
+"""Autotuner for the launch shape of the split-limb MFMA scoring kernel.
 
 The kernel has two shape parameters (gnm_set_launch_config):
   * tiles_per_pass -- 16-column weight/count tiles one workgroup accumulates
-    (3, 4, 5, 6 or 8).  More tiles per workgroup decode each .bed stage once for
-    more columns but hold more accumulators (fewer waves per SIMD); fewer
-    tiles give more workgroups per CU and more decode work.  All tile groups
-    run in one launch either way.
-  * batch_stages -- 64-row stages whose loads are issued together (2 or 4).
+    (split-int8: 3, 4, 5, 6, 8 or the two-set 18; split-BF16: 3, 5, 6, 8 or
+    9).  More tiles per workgroup decode each .bed stage once for more
+    columns but hold more accumulators (fewer waves per SIMD); fewer tiles
+    give more workgroups per CU and more decode work.  All tile groups run
+    in one launch either way.
+  * batch_stages -- 64-row stages whose loads are issued together (2 or 4
+    on the split-int8 path, 2 on the split-BF16 path).
 
 Which shape wins depends on the number of people, the number of tiles
 (scores x limbs + membership classes) and the row pattern, so the tuner
 measures: each candidate scores the same sample block of rows on the
 context itself (HIP-event time of the main kernel, best of `reps`), and the
-fastest is kept.  The context must be fresh (no rows scored yet); its
-accumulators are restored exactly afterwards (they hold only the baseline
-term, b - b = 0).  Choices are remembered per shape class in memory and,
-when a path is given, in a JSON file.
+fastest is kept.  The context must be fresh (no rows scored yet); before
+every run its accumulators are seeded back to that state (they hold only the
+baseline term, b - b = 0), so each run is a first pass over the sample and
+the context is left exactly as it was.  Choices are remembered per shape
+class in memory and, when a path is given, in a JSON file.
 """
 from __future__ import annotations
 
@@ -23,6 +26,12 @@ import json
 import os
 
 CANDIDATES = [(3, 4), (3, 2), (4, 4), (4, 2), (5, 4), (5, 2), (6, 4), (6, 2), (8, 4), (8, 2), (18, 2)]
+BF16_CANDIDATES = [(3, 2), (5, 2), (6, 2), (8, 2), (9, 2)]
+
+
+def candidates_for(precision: str):
+    return CANDIDATES if precision.startswith("i8") or precision == "auto" else BF16_CANDIDATES
+
 
 _memory: dict[str, tuple[int, int]] = {}
 
@@ -41,10 +50,11 @@ def _load(path):
     return {}
 
 
-def tune(ctx, d_rows_ptr: int, row_stride: int, num_rows: int, first_variant: int = 0, reps: int = 2,
-         cache_path: str | None = None, candidates=CANDIDATES, log=None) -> dict:
-    """Pick the fastest launch shape for `ctx` on rows [first_variant,
-    first_variant + num_rows) of a device-resident .bed and apply it."""
+def tune(ctx, d_rows_ptr: int, row_stride: int, num_rows: int, reps: int = 2,
+         cache_path: str | None = None, candidates=None, log=None) -> dict:
+    """Pick the fastest launch shape for `ctx` on rows [0, num_rows) of a
+    device-resident .bed and apply it."""
+    candidates = candidates or candidates_for(ctx.precision)
     key = shape_key(ctx.N, ctx.K, ctx.precision, ctx.V)
     cache = _load(cache_path)
     if key in _memory or key in cache:
@@ -59,7 +69,8 @@ def tune(ctx, d_rows_ptr: int, row_stride: int, num_rows: int, first_variant: in
         ctx.set_launch_config(tp, bs)
         best = None
         for r in range(reps + 1):           # the first run warms caches
-            ctx.score_rows_device(d_rows_ptr, row_stride, first_variant, num_rows)
+            ctx.seed_accumulators(S0, C0, 0)
+            ctx.score_rows_device(d_rows_ptr, row_stride, 0, num_rows)
             main_ms, _ = ctx.last_timing()
             if r:
                 best = main_ms if best is None else min(best, main_ms)

@@ -638,6 +638,13 @@ gnm_status check_block(gnm_ctx *c, uint64_t first_variant, uint64_t num_rows, ui
     if (!c) return fail(GNM_ERR_INVALID_ARG, "ctx is NULL");
     if (first_variant > c->V || num_rows > c->V - first_variant)
         return fail(GNM_ERR_INVALID_ARG, "row block exceeds num_variants");
+    // blocks arrive in ascending order without gaps or repeats (the
+    // producer's order, score/io.rs:156-321): a block starts where the last
+    // one ended (after a seed, at the seeded count), or at variant 0 once
+    // every variant has been scored (a further pass accumulates on top)
+    if (first_variant != c->completed && !(first_variant == 0 && c->completed == c->V))
+        return fail(GNM_ERR_INVALID_ARG, "row block starts at variant " + std::to_string(first_variant) +
+                                             " but the next variant to score is " + std::to_string(c->completed));
     if (row_stride < c->bpv) return fail(GNM_ERR_INVALID_ARG, "row_stride < bytes_per_variant");
     return GNM_OK;
 }

@@ -32,14 +32,17 @@ and exponents carry no ``+`` and no leading zeros.  The shortest digits come
 from Python's float repr (f64) and numpy's unique float32 formatting, both
 shortest-round-trip algorithms like ryu's.
 
-The file is written to a temporary name in the target directory and renamed
-into place (main.rs:1409-1436, 1541-1547).
+The writer never leaves a partial file at the target path: the lines go to a
+fresh temporary file that tempfile creates next to the target, which is
+synced and then atomically renamed over it (the reference likewise writes
+through a temporary file and a rename, main.rs:1409-1436, 1541-1547; the
+naming and error handling here are this module's own).
 """
 from __future__ import annotations
 
 import math
 import os
-import time
+import tempfile
 from typing import Iterable, Mapping, Sequence
 
 import numpy as np
@@ -102,7 +105,8 @@ def sscore_lines(iids: Sequence[str], score_names: Sequence[str], score_variant_
     missing_counts = np.asarray(missing_counts, dtype=np.uint32)
     K = len(score_names)
     if sums.shape != (len(iids), K) or missing_counts.shape != (len(iids), K):
-        raise ValueError("Mismatched number of persons and score rows during final write.")
+        raise ValueError(f"sums and missing_counts must both be {len(iids)} x {K} (people x scores); got "
+                         f"{sums.shape} and {missing_counts.shape}")
     if regions:
         header_done = False
         for name in score_names:
@@ -119,33 +123,27 @@ def sscore_lines(iids: Sequence[str], score_names: Sequence[str], score_variant_
 
 
 def write_sscore(path, iids, score_names, score_variant_counts, sums, missing_counts, regions=None) -> None:
-    """Write the .sscore file at ``path`` through a temporary file in the same
-    directory that is synced and renamed into place."""
+    """Write the .sscore file at ``path``.  The text is formatted completely
+    before the target directory is touched, written to a temporary file in
+    that directory, synced, and renamed onto ``path``; on any failure the
+    temporary file is removed and ``path`` is left as it was."""
     path = os.fspath(path)
-    out_dir = os.path.dirname(path) or "."
-    name = os.path.basename(path)
-    if not name:
-        raise ValueError(f"Output path '{path}' has no file name.")
-    stamp = time.time_ns()
-    tmp = None
-    for attempt in range(32):
-        cand = os.path.join(out_dir, f".{name}.{os.getpid()}.{stamp + attempt}.tmp")
-        try:
-            fd = os.open(cand, os.O_WRONLY | os.O_CREAT | os.O_EXCL, 0o644)
-        except FileExistsError:
-            continue
-        tmp = cand
-        break
-    if tmp is None:
-        raise FileExistsError(f"Failed to allocate a unique temporary output file in '{out_dir}'.")
+    target_dir, base = os.path.split(os.path.abspath(path))
+    if not os.path.basename(path):
+        raise IsADirectoryError(f"{path!r} names a directory, not an output file")
+    text = "".join(line + "\n" for line in sscore_lines(iids, score_names, score_variant_counts, sums,
+                                                        missing_counts, regions))
+    fd, tmp = tempfile.mkstemp(prefix=f".{base}.", suffix=".partial", dir=target_dir)
     try:
         with os.fdopen(fd, "w", encoding="utf-8", newline="\n") as f:
-            for line in sscore_lines(iids, score_names, score_variant_counts, sums, missing_counts, regions):
-                f.write(line + "\n")
+            f.write(text)
             f.flush()
             os.fsync(f.fileno())
+        os.chmod(tmp, 0o644)
         os.replace(tmp, path)
     except BaseException:
-        if os.path.exists(tmp):
-            os.remove(tmp)
+        try:
+            os.unlink(tmp)
+        except FileNotFoundError:
+            pass
         raise

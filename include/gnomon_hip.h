@@ -127,7 +127,11 @@ gnm_status gnm_create_forced(const gnm_problem *problem, gnm_ctx **out_ctx);
  * run_pending_compute_cuda (cuda_backend.rs:1827-1978, 2067-2333).
  * rows: num_rows packed .bed rows (bytes_per_variant each, row stride given)
  * for reconciled variants [first_variant, first_variant + num_rows), in
- * ascending order, on the host.  The call copies the rows through pinned
+ * ascending order, on the host.  Blocks follow each other without gaps or
+ * repeats: first_variant must be the number of variants completed so far
+ * (after gnm_seed_accumulators, the seeded count), or 0 once all
+ * num_variants are done (a further pass then accumulates on top);
+ * GNM_ERR_INVALID_ARG otherwise.  The call copies the rows through pinned
  * staging and returns once they have been consumed (the caller may reuse
  * the buffer); compute stays asynchronous. */
 gnm_status gnm_score_rows(gnm_ctx *ctx, const uint8_t *rows, uint64_t row_stride,

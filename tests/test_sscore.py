@@ -121,7 +121,7 @@ def test_file_layout_and_values(tmp_path):
     text = path.read_text()
     assert text.startswith("#REGION\tSCORE\tINTERVAL\n#REGION\tPGS_C\tchr1:100-200\n#IID\tPGS_A_AVG\t")
     assert text.endswith("\n") and "\r" not in text
-    assert [f for f in os.listdir(tmp_path) if f.endswith(".tmp")] == []   # temp file renamed away
+    assert [f for f in os.listdir(tmp_path) if f.endswith(".partial")] == []   # temp file renamed away
     got_names, rows = _parse_sscore_table(text)
     assert got_names == names and list(rows) == iids
     avg, pct = sscore_columns(sums, miss, counts_k)
