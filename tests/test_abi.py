@@ -132,3 +132,46 @@ def test_precision_codes():
     pr = _problem(csr, 10, precision=7)
     assert L.gnm_create_forced(ctypes.byref(pr), ctypes.byref(ctx)) == _lib.GNM_ERR_INVALID_ARG
     assert "unknown precision" in _lib.last_error()
+
+
+def _rust_string_list(text, const_name):
+    m = re.search(const_name + r"\s*:\s*\[&str;\s*(\d+)\]\s*=\s*\[(.*?)\];", text, re.S)
+    assert m, const_name
+    items = re.findall(r'"([^"]+)"', m.group(2))
+    assert len(items) == int(m.group(1))
+    return items
+
+
+def test_rust_build_lists_every_source():
+    """score/hip_build.rs and INTEGRATION.md compile what gnomon_amd/build.py
+    compiles, with the same flags, so the Rust build links with no undefined
+    symbols."""
+    from gnomon_amd import build
+
+    rs = open(os.path.join(ROOT, "score", "hip_build.rs")).read()
+    assert _rust_string_list(rs, "SOURCES") == list(build.SOURCE_NAMES)
+    assert _rust_string_list(rs, "COMPILE_FLAGS") == build.HIPCC_FLAGS + ["-c"]
+    assert _rust_string_list(rs, "LINK_FLAGS") == build.LINK_FLAGS
+    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    listed = re.search(r"in link order:\s*```text\s*\n(.*?)\n```", doc, re.S)
+    assert listed and listed.group(1).split() == list(build.SOURCE_NAMES)
+    on_disk = sorted(f for f in os.listdir(os.path.join(ROOT, "gnomon_amd", "csrc")) if f.endswith(".hip"))
+    assert sorted(build.SOURCE_NAMES) == on_disk
+
+
+def test_rust_shim_declares_header_functions():
+    """Every function score/hip_backend.rs binds is declared in the header and
+    exported by the library; GnmProblem has gnm_problem's fields in order."""
+    rs = open(os.path.join(ROOT, "score", "hip_backend.rs")).read()
+    block = re.search(r'extern "C"\s*\{(.*?)\n\}', rs, re.S).group(1)
+    bound = re.findall(r"fn (gnm_\w+)\s*\(", block)
+    assert bound and set(bound) <= set(_declared())
+    called = set(re.findall(r"\b(gnm_\w+)\s*\(", rs.replace(block, "")))
+    assert called <= set(bound), called - set(bound)
+    hdr = open(os.path.join(ROOT, "include", "gnomon_hip.h")).read()
+    c_struct = re.search(r"typedef struct gnm_problem \{(.*?)\} gnm_problem;", hdr, re.S).group(1)
+    c_fields = re.findall(r"\b(\w+)(?:\[\d+\])?;", re.sub(r"/\*.*?\*/", "", c_struct, flags=re.S))
+    r_struct = re.search(r"struct GnmProblem \{(.*?)\}", rs, re.S).group(1)
+    r_fields = re.findall(r"(\w+)\s*:", r_struct)
+    assert r_fields == c_fields
+    assert [f for f, _ in _lib.Problem._fields_] == c_fields
