@@ -34,7 +34,8 @@ EXPORTS = (
     "gnm_last_error", "gnm_abi_version", "gnm_size_gate", "gnm_device_count", "gnm_create",
     "gnm_create_forced", "gnm_score_rows", "gnm_score_rows_device", "gnm_read_accumulators",
     "gnm_seed_accumulators", "gnm_finish", "gnm_destroy", "gnm_run", "gnm_unpack_device",
-    "gnm_dense_forward_device", "gnm_dense_adjoint_device", "gnm_synth_rows_device",
+    "gnm_dense_forward_device", "gnm_dense_adjoint_device", "gnm_dense_adjoint_device_noalloc",
+    "gnm_synth_rows_device",
     "gnm_last_timing", "gnm_set_launch_config", "gnm_set_sparse_ingest", "gnm_ingest_stats",
     "gnm_sparse_encode_row",
 )
@@ -109,8 +110,10 @@ def lib():
     L.gnm_unpack_device.argtypes = [vp, u64, u64, u64, vp, vp, vp, vp]
     L.gnm_dense_forward_device.restype = st
     L.gnm_dense_forward_device.argtypes = [vp, u64, u64, u64, vp, vp, u64, i32, vp, vp, vp]
-    L.gnm_dense_adjoint_device.restype = st
-    L.gnm_dense_adjoint_device.argtypes = [vp, u64, u64, u64, vp, vp, u64, i32, vp, vp]
+    for name in ("gnm_dense_adjoint_device", "gnm_dense_adjoint_device_noalloc"):
+        f = getattr(L, name)
+        f.restype = st
+        f.argtypes = [vp, u64, u64, u64, vp, vp, u64, i32, vp, vp]
     L.gnm_synth_rows_device.restype = st
     L.gnm_synth_rows_device.argtypes = [u64, u32, u32, u32, u64, u64, u64, vp, u64, vp]
     L.gnm_set_launch_config.restype = st

@@ -1226,6 +1226,47 @@ gnm_status ws_typed(DenseWorkspace *ws, size_t slot, size_t count, hipStream_t s
     return s;
 }
 
+// Buffers of the split-int8 adjoint, in elements: block exponents (int),
+// R limb fragments (u32), limb scales (f64); the FP64 accumulator is
+// [V][Kp].  R is split per block of kI8FlushRows people.
+struct AdjointI8Shape {
+    uint32_t L, used, tiles;
+    uint64_t Kp, nks, nblk, exps, frag, scales;
+};
+
+gnm_status adjoint_i8_shape(uint64_t N, uint64_t K, int32_t precision, bool aligned, AdjointI8Shape *sh) {
+    sh->L = limbs_of(precision);
+    sh->Kp = (K + 15) / 16 * 16;
+    sh->used = (uint32_t)(sh->L * (sh->Kp / 16));
+    sh->tiles = (sh->used + 3) / 4 * 4;
+    if (sh->tiles > 16) return fail(GNM_ERR_UNSUPPORTED, "the int8 adjoint supports up to 64 scores per call");
+    if (!aligned)
+        return fail(GNM_ERR_UNSUPPORTED, "the int8 adjoint needs 4-byte aligned rows with a stride that is a "
+                                         "multiple of 4 (below 2^25)");
+    const uint64_t nss = ((N + 3) / 4 + 63) / 64;   // super-steps of 256 people
+    sh->nks = nss * 4;
+    sh->nblk = (N + kI8FlushRows - 1) / kI8FlushRows;
+    sh->exps = sh->nblk * K;
+    sh->frag = sh->nks * sh->tiles * 64 * 4;
+    sh->scales = sh->nblk * sh->L * sh->Kp;
+    return GNM_OK;
+}
+
+// Every buffer is fully written before it is read: scales are cleared here,
+// exponents and fragments are written whole, and the adjoint kernel's first
+// flush per variant overwrites the accumulator.
+gnm_status adjoint_i8_launch(const AdjointI8Shape &sh, const uint8_t *d_rows, uint64_t stride, uint64_t V,
+                             uint64_t N, const uint8_t *d_flip, const float *d_grad_scores, uint64_t K, int *exps,
+                             uint32_t *frag, double *scales, double *acc, float *d_grad_weights, hipStream_t st) {
+    GNM_HIP(launch_fill_f64(scales, sh.scales, 0.0, st));
+    GNM_HIP(launch_block_exponent(d_grad_scores, N, K, K, kI8FlushRows, sh.L, true, exps, st));
+    GNM_HIP(launch_adjoint_frag_i8(d_grad_scores, N, K, sh.Kp, sh.L, exps, sh.tiles, sh.used, sh.nks, frag, scales,
+                                   st));
+    GNM_HIP(launch_adjoint_i8(d_rows, stride, V, N, d_flip, frag, scales, sh.tiles, K, sh.Kp, sh.L, acc,
+                              d_grad_weights, st));
+    return GNM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1353,37 +1394,22 @@ gnm_status gnm_dense_adjoint_device(const uint8_t *d_rows, uint64_t bytes_per_va
                                                                              : GNM_PRECISION_BF16X3;
         }
         if (is_int8(precision)) {
-            // split-int8 adjoint: R limbs per block of kI8FlushRows people
-            const uint32_t L = limbs_of(precision);
-            const uint64_t Kp = (K + 15) / 16 * 16;
-            const uint32_t used = (uint32_t)(L * (Kp / 16));
-            const uint32_t tiles = (used + 3) / 4 * 4;
-            if (tiles > 16) return fail(GNM_ERR_UNSUPPORTED, "the int8 adjoint supports up to 64 scores per call");
-            if (!aligned)
-                return fail(GNM_ERR_UNSUPPORTED,
-                            "the int8 adjoint needs 4-byte aligned rows with a stride that is a multiple of 4 "
-                            "(below 2^25)");
+            AdjointI8Shape sh;
+            gnm_status s = adjoint_i8_shape(N, K, precision, aligned, &sh);
+            if (s != GNM_OK) return s;
             hipStream_t st = (hipStream_t)stream_;
             DenseWorkspace *ws = workspace_for_current_device();
             if (!ws) return fail(GNM_ERR_NO_DEVICE, "no current HIP device");
             std::lock_guard<std::mutex> lk(ws->mu);
-            const uint64_t nss = ((N + 3) / 4 + 63) / 64;       // super-steps of 256 people
-            const uint64_t nks = nss * 4;
-            const uint64_t nblk = (N + kI8FlushRows - 1) / kI8FlushRows;
-            gnm_status s;
             int *exps;
             uint32_t *frag;
             double *scales, *acc;
-            if ((s = ws_typed(ws, 11, nblk * K, st, &exps)) != GNM_OK) return s;
-            if ((s = ws_typed(ws, 12, nks * tiles * 64 * 4, st, &frag)) != GNM_OK) return s;
-            if ((s = ws_typed(ws, 13, nblk * L * Kp, st, &scales)) != GNM_OK) return s;
-            if ((s = ws_typed(ws, 15, std::max<uint64_t>(V, 1) * Kp, st, &acc)) != GNM_OK) return s;
-            GNM_HIP(launch_fill_f64(scales, nblk * L * Kp, 0.0, st));
-            GNM_HIP(launch_block_exponent(d_grad_scores, N, K, K, kI8FlushRows, L, true, exps, st));
-            GNM_HIP(launch_adjoint_frag_i8(d_grad_scores, N, K, Kp, L, exps, tiles, used, nks, frag, scales, st));
-            GNM_HIP(launch_adjoint_i8(d_rows, bytes_per_variant, V, N, d_flip, frag, scales, tiles, K, Kp, L, acc,
-                                      d_grad_weights, st));
-            return GNM_OK;
+            if ((s = ws_typed(ws, 11, sh.exps, st, &exps)) != GNM_OK) return s;
+            if ((s = ws_typed(ws, 12, sh.frag, st, &frag)) != GNM_OK) return s;
+            if ((s = ws_typed(ws, 13, sh.scales, st, &scales)) != GNM_OK) return s;
+            if ((s = ws_typed(ws, 15, std::max<uint64_t>(V, 1) * sh.Kp, st, &acc)) != GNM_OK) return s;
+            return adjoint_i8_launch(sh, d_rows, bytes_per_variant, V, N, d_flip, d_grad_scores, K, exps, frag,
+                                     scales, acc, d_grad_weights, st);
         }
         if (precision != GNM_PRECISION_BF16X2 && precision != GNM_PRECISION_BF16X3 && precision != GNM_PRECISION_BF16X4)
             return fail(GNM_ERR_UNSUPPORTED, "the adjoint runs in split-BF16 (bf16x2, bf16x3, bf16x4) or "
@@ -1416,6 +1442,90 @@ gnm_status gnm_dense_adjoint_device(const uint8_t *d_rows, uint64_t bytes_per_va
         return GNM_OK;
     } catch (...) {
         return fail(GNM_ERR_INTERNAL, "exception in gnm_dense_adjoint_device");
+    }
+}
+
+}  // extern "C"
+
+namespace {
+
+// Fixed device scratch of gnm_dense_adjoint_device_noalloc.  Every GPU the
+// library runs on holds its own copy of this array from the first use of this
+// file's code on that GPU until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in for the symbol):
+// its device address is looked up once per device with hipGetSymbolAddress
+// and kept in g_scratch_ptr.
+constexpr size_t kNoAllocScratchBytes = 96ull << 20;
+constexpr int kNoAllocMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+gnm_status scratch_for_current_device(void **out) {
+    int dev = 0;
+    GNM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return fail(GNM_ERR_UNSUPPORTED, "device ordinal above 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(GNM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return GNM_OK;
+}
+
+uint64_t align256(uint64_t b) { return (b + 255) / 256 * 256; }
+
+}  // namespace
+
+extern "C" {
+
+gnm_status gnm_dense_adjoint_device_noalloc(const uint8_t *d_rows, uint64_t bytes_per_variant,
+                                            uint64_t num_variants, uint64_t num_people, const uint8_t *d_flip,
+                                            const float *d_grad_scores, uint64_t num_scores, int32_t precision,
+                                            float *d_grad_weights, void *stream_) {
+    try {
+        const uint64_t V = num_variants, N = num_people, K = num_scores;
+        if (N == 0 || K == 0) return fail(GNM_ERR_INVALID_ARG, "num_people and num_scores must be positive");
+        if (V && (!d_rows || !d_grad_weights)) return fail(GNM_ERR_INVALID_ARG, "NULL device pointer");
+        if (!d_grad_scores) return fail(GNM_ERR_INVALID_ARG, "NULL grad_scores");
+        if (bytes_per_variant < (N + 3) / 4) return fail(GNM_ERR_INVALID_ARG, "bytes_per_variant too small");
+        if (!is_int8(precision))
+            return fail(GNM_ERR_UNSUPPORTED, "the allocation-free adjoint runs in split-int8 (i8x3, i8x4) only");
+        const bool aligned = ((reinterpret_cast<uintptr_t>(d_rows) | bytes_per_variant) & 3u) == 0 &&
+                             bytes_per_variant < (1ull << 25);
+        AdjointI8Shape sh;
+        gnm_status s = adjoint_i8_shape(N, K, precision, aligned, &sh);
+        if (s != GNM_OK) return s;
+        const uint64_t b_exps = align256(sh.exps * sizeof(int));
+        const uint64_t b_frag = align256(sh.frag * sizeof(uint32_t));
+        const uint64_t b_scales = align256(sh.scales * sizeof(double));
+        const uint64_t b_acc = align256(std::max<uint64_t>(V, 1) * sh.Kp * sizeof(double));
+        const uint64_t need = b_exps + b_frag + b_scales + b_acc;
+        if (need > kNoAllocScratchBytes)
+            return fail(GNM_ERR_INVALID_ARG,
+                        "problem needs " + std::to_string(need) + " bytes of scratch; the allocation-free adjoint "
+                        "has " + std::to_string(kNoAllocScratchBytes) + " (about 8*V*round16(K) + "
+                        "16*N*round4(limbs*round16(K)/16) bytes are needed)");
+        if (V == 0) return GNM_OK;
+        void *base = nullptr;
+        if ((s = scratch_for_current_device(&base)) != GNM_OK) return s;
+        unsigned char *p = (unsigned char *)base;
+        int *exps = (int *)p;
+        uint32_t *frag = (uint32_t *)(p + b_exps);
+        double *scales = (double *)(p + b_exps + b_frag);
+        double *acc = (double *)(p + b_exps + b_frag + b_scales);
+        return adjoint_i8_launch(sh, d_rows, bytes_per_variant, V, N, d_flip, d_grad_scores, K, exps, frag, scales,
+                                 acc, d_grad_weights, (hipStream_t)stream_);
+    } catch (...) {
+        return fail(GNM_ERR_INTERNAL, "exception in gnm_dense_adjoint_device_noalloc");
     }
 }
 

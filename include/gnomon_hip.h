@@ -235,6 +235,24 @@ gnm_status gnm_dense_adjoint_device(const uint8_t *d_rows, uint64_t bytes_per_va
                                     uint64_t num_scores, int32_t precision,
                                     float *d_grad_weights, void *stream);
 
+/* gnm_dense_adjoint_device without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving or training
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * Split-int8 only (I8X3 / I8X4; other precisions GNM_ERR_UNSUPPORTED), same
+ * operands, kernels and results as gnm_dense_adjoint_device at that
+ * precision.  Its scratch (block exponents, R limb fragments, limb scales and
+ * an FP64 [V][round16(K)] accumulator) lives in a fixed 96 MiB device array
+ * per GPU, held from the library's first use on that GPU until the process
+ * ends; problems needing more (about 8*V*round16(K) bytes plus 16 bytes per
+ * person per 16-column limb tile) get GNM_ERR_INVALID_ARG before any launch.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (different streams or threads without ordering) race. */
+gnm_status gnm_dense_adjoint_device_noalloc(const uint8_t *d_rows, uint64_t bytes_per_variant,
+                                            uint64_t num_variants, uint64_t num_people,
+                                            const uint8_t *d_flip, const float *d_grad_scores,
+                                            uint64_t num_scores, int32_t precision,
+                                            float *d_grad_weights, void *stream);
+
 /* Synthetic SNP-major rows on the device, bit-identical to the test oracle's
  * generator (variant v, sample s keyed counter-based hash; HWE with A2
  * frequency in [p_lo24, p_hi24] * 2^-24; missing with probability
