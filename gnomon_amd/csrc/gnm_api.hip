@@ -452,12 +452,12 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
         GNM_HIP(launch_plan_rows(d_off, d_cols, d_w, d_co, d_prog_off, d_var_rows, V, K, c->d_score_class, cw, d_wrow,
                                  d_rowcls, c->stream));
         if (!mfma_weights) {
-            const uint64_t k32 = (K + kFp32ScoreChunk - 1) / kFp32ScoreChunk * kFp32ScoreChunk;
-            double *d_w64;
-            if ((st = c->alloc(&d_w64, Rpad * k32)) != GNM_OK) return st;
-            GNM_HIP(launch_fill_f64(d_w64, Rpad * k32, 0.0, c->stream));
-            GNM_HIP(launch_plan_w64(d_wrow, R, K, k32, d_w64, c->stream));
-            pl.w64 = d_w64;
+            const uint64_t k32 = fp32_weight_pitch(K);
+            float *d_w32;
+            if ((st = c->alloc(&d_w32, Rpad * k32)) != GNM_OK) return st;
+            GNM_HIP(launch_fill_u32((uint32_t *)d_w32, Rpad * k32, 0u, c->stream));
+            GNM_HIP(launch_plan_w32(d_wrow, R, K, k32, d_w32, c->stream));
+            pl.w32 = d_w32;
             pl.k32 = k32;
         }
         if (tiles) {
@@ -586,7 +586,7 @@ gnm_status score_device_rows(gnm_ctx *c, const uint8_t *d_rows, uint64_t stride,
     GNM_HIP(hipEventRecord(c->ev_main0, c->stream));
     if (c->precision == GNM_PRECISION_FP32) {
         GNM_HIP(launch_score_fp32(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, c->K, S,
-                                  c->fp32_count_in_kernel, c->d_part, c->d_cpart, c->stream));
+                                  c->fp32_count_in_kernel, identity, c->d_part, c->d_cpart, c->stream));
         if (!c->fp32_count_in_kernel)
             GNM_HIP(launch_score_bf16(c->plan, rows, rstride, bpv, v0, p0, p1, c->N, S, 0, c->plan.ctiles, identity,
                                       c->d_part, c->d_cpart, c->stream));
@@ -1323,13 +1323,13 @@ gnm_status gnm_dense_forward_device(const uint8_t *d_rows, uint64_t bytes_per_va
         GNM_HIP(launch_fill_f64(d_scores, N * K, 0.0, st));
         GNM_HIP(launch_fill_u32(cnt_tmp, N * K, 0u, st));
         if (precision == GNM_PRECISION_FP32) {
-            const uint64_t k32 = (K + kFp32ScoreChunk - 1) / kFp32ScoreChunk * kFp32ScoreChunk;
-            double *w64;
-            if ((s = ws_typed(ws, 7, Rpad * k32, st, &w64)) != GNM_OK) return s;
-            GNM_HIP(launch_pad_weights(d_weights, V, K, Rpad, k32, w64, st));
-            pl.w64 = w64;
+            const uint64_t k32 = fp32_weight_pitch(K);
+            float *w32;
+            if ((s = ws_typed(ws, 7, Rpad * k32, st, &w32)) != GNM_OK) return s;
+            GNM_HIP(launch_pad_weights(d_weights, V, K, Rpad, k32, w32, st));
+            pl.w32 = w32;
             pl.k32 = k32;
-            GNM_HIP(launch_score_fp32(pl, d_rows, bytes_per_variant, (N + 3) / 4, 0, 0, V, N, K, S, true,
+            GNM_HIP(launch_score_fp32(pl, d_rows, bytes_per_variant, (N + 3) / 4, 0, 0, V, N, K, S, true, true,
                                       part, cpart, st));
         } else {
             pl.ncol = (uint32_t)(L * K);

@@ -170,14 +170,14 @@ hipError_t launch_dense_program(const uint8_t *flip, uint64_t V, uint64_t Rpad, 
 
 // W [V][K] -> W32 [Rpad][k32] zero padded (FP32 kernel layout)
 __global__ void k_pad_weights(const float *__restrict__ W, uint64_t V, uint64_t K, uint64_t Rpad,
-                              uint64_t k32, double *__restrict__ out) {
+                              uint64_t k32, float *__restrict__ out) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Rpad * k32) return;
     const uint64_t r = i / k32, k = i % k32;
-    out[i] = (r < V && k < K) ? (double)W[r * K + k] : 0.0;
+    out[i] = (r < V && k < K) ? W[r * K + k] : 0.0f;
 }
 
-hipError_t launch_pad_weights(const float *W, uint64_t V, uint64_t K, uint64_t Rpad, uint64_t k32, double *out,
+hipError_t launch_pad_weights(const float *W, uint64_t V, uint64_t K, uint64_t Rpad, uint64_t k32, float *out,
                               hipStream_t stream) {
     const uint64_t n = Rpad * k32;
     if (n == 0) return hipSuccess;

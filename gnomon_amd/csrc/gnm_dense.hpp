@@ -17,7 +17,7 @@ hipError_t launch_forward_frag(const float *W, uint64_t V, uint64_t K, uint32_t 
 hipError_t launch_dense_program(const uint8_t *flip, uint64_t V, uint64_t Rpad, uint32_t *prog_var,
                                 uint32_t *flipmask, uint32_t *prog_cls, hipStream_t stream);
 hipError_t launch_pad_weights(const float *W, uint64_t V, uint64_t K, uint64_t Rpad, uint64_t k32,
-                              double *out, hipStream_t stream);
+                              float *out, hipStream_t stream);
 hipError_t launch_take_col0(const uint32_t *counts, uint64_t N, uint64_t K, uint32_t *out,
                             hipStream_t stream);
 hipError_t launch_adjoint_frag(const float *R, uint64_t N, uint64_t K, uint64_t Kp, uint32_t L,

@@ -25,7 +25,6 @@ constexpr int kI8FlushRows = 65536;
 constexpr int kBfFlushRows = 32768;
 constexpr int flush_rows(bool int8) { return int8 ? kI8FlushRows : kBfFlushRows; }
 constexpr int kLimbBits = 8;
-constexpr int kFp32ScoreChunk = 8;     // scores per pass of the FP32 kernel
 constexpr int kFp32MaxClasses = 4;     // membership classes counted in-kernel by the FP32 kernel
 
 // Signed base-256 digits of an integer with |Q| <= 2^(8L) - 2^(8(L-1)):
@@ -75,7 +74,7 @@ struct DevicePlan {
     const uint32_t *flipmask;  // [ceil(R/32)] bit j of word s: row 32s+j reads the A1 allele
     const uint32_t *prog_cls;  // [R] membership-class bitmask of count rows (FP32 kernel, <=32 classes)
     // FP32 path
-    const double *w64;         // [R][k32] effect-allele FP32 weights held as f64 (k32 = K rounded up to 8)
+    const float *w32;          // [Rpad][k32] effect-allele FP32 weights (k32 = fp32_weight_pitch(K))
     uint64_t k32;
     // split-BF16 path
     const uint32_t *bfrag;     // [R/32][tiles][64 lanes][4 dwords] B fragments (weights then counts)
@@ -103,10 +102,15 @@ hipError_t launch_compact(const uint8_t *rows, uint64_t stride, uint64_t num_row
                           const uint32_t *out_to_fam, uint64_t num_people, uint8_t *out,
                           uint64_t out_stride, hipStream_t stream);
 uint32_t fp32_splits(uint64_t num_people, uint64_t num_scores);
+// row pitch (in floats) of the FP32 kernel's weight table for num_scores scores
+uint64_t fp32_weight_pitch(uint64_t num_scores);
+// Plain FP32 scoring (FP32 matrix cores).  identity: program row p0 + i is
+// row i of the block (no per-row variant lookup).
 hipError_t launch_score_fp32(const DevicePlan &plan, const uint8_t *rows, uint64_t stride,
                              uint64_t bpv, uint64_t v0, uint64_t p0, uint64_t p1,
                              uint64_t num_people, uint64_t num_scores, uint32_t splits,
-                             bool count_in_kernel, double *part, uint32_t *cpart, hipStream_t stream);
+                             bool count_in_kernel, bool identity, double *part, uint32_t *cpart,
+                             hipStream_t stream);
 // Split-int8 scoring (gnm_i8.hip); rows must be 4-byte aligned with a
 // 4-byte multiple stride below 2^25 for the dword staging loads (else byte
 // loads).  One launch covers every tile group of [tile_begin, tile_end)
@@ -167,8 +171,8 @@ hipError_t launch_plan_frag(bool int8, const double *wrow, uint64_t R, uint64_t 
                             uint32_t L, uint32_t ncol, uint32_t wtiles, uint32_t tiles, uint64_t nks,
                             const uint8_t *count_row, const uint32_t *rowcls, uint32_t cw, uint32_t nclass,
                             void *frag, hipStream_t stream);
-// FP32-path weights w64[r][k] (row pitch k32) = W rounded to f32
-hipError_t launch_plan_w64(const double *wrow, uint64_t R, uint64_t K, uint64_t k32, double *w64, hipStream_t stream);
+// FP32-path weights w32[r][k] (row pitch k32) = W rounded to f32
+hipError_t launch_plan_w32(const double *wrow, uint64_t R, uint64_t K, uint64_t k32, float *w32, hipStream_t stream);
 
 // Sparse ingest (gnm_sparse.hip).  A sparse row goes as a bitmap of its
 // non-zero 64-bit words (32 calls each) plus those words, in chunks of

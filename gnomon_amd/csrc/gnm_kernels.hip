@@ -368,123 +368,278 @@ hipError_t launch_compact(const uint8_t *rows, uint64_t stride, uint64_t num_row
 }
 
 // ---------------------------------------------------------------------------
-// Plain FP32 path: a wavefront reduction with people on lanes (four per lane,
-// one packed byte) and the row's weights broadcast from scalar registers.
-// FP32 weights, exact products (dosage 0/1/2 times an FP32 weight is exact in
-// FP64 as in FP32) and FP64 accumulation per genotype: v_fma_f64 runs at the
-// same rate as unpacked v_fma_f32 on CDNA4, and it removes the FP32 partial-sum
-// rounding of the reference kernel's 256-variant mini-batches
-// (score/kernel.rs:124-176, score/batch.rs:34, 387-419), which alone exceeds
-// 1e-6 relative at 1e5-1e6 variants.
+// Plain FP32 path on the FP32 matrix cores (v_mfma_f32_4x4x1_16b_f32).
+//
+// Arithmetic: FP32 weights, exact products (dosage 0/1/2 times an FP32 weight
+// is exact in FP32), FP32 accumulation over blocks of kF32FlushRows program
+// rows, each block sum then added in FP64 -- the arithmetic class of the
+// reference kernel's FP32 sums over 256-variant mini-batches widened into
+// FP64 accumulators (score/kernel.rs:124-176, score/batch.rs:34, 387-419).
+//
+// Layout: a thread owns one byte column of the rows (four people, one per
+// "tile" t = the person's position in the byte); a workgroup of 4 waves reads
+// 256 contiguous bytes of every row.  One MFMA multiplies 16 blocks of
+// 4 x 1 by 1 x 4: the B operand is each lane's own dosage (the lane is one
+// person of the block), the A operand holds the weights of 16 rows -- lane
+// 4q + i carries score 4c + i of row q -- and the broadcast modifier
+// (CBSZ 4, ABID q) hands row q's four weights to all 16 blocks.  Lane l then
+// accumulates score 4c + i of its person in output register i.  One A
+// register serves 16 rows, so the weights cost one load per 16 rows per
+// four scores.
+//
+// Decode, per lane and row: the four 2-bit codes are spread into bytes
+// (two shift-or steps), the row's flip selector is OR-ed in, and one
+// v_perm_b32 maps (flip, code) to four dosage bytes, another to four missing
+// flags; the dosage bytes convert to FP32 with v_cvt_f32_ubyte{0..3}.
+// Missing calls are counted as packed bytes, at most kF32FlushRows per block.
 // ---------------------------------------------------------------------------
-template <int KC>
-__global__ __launch_bounds__(256) void k_score_fp32(DevicePlan plan, const uint8_t *__restrict__ rows,
-                                                    uint64_t stride, uint64_t bpv, uint64_t v0,
-                                                    uint64_t p0, uint64_t p1, uint64_t split_base,
-                                                    uint64_t rps, uint64_t num_people,
-                                                    uint64_t num_scores, int count_in_kernel,
-                                                    double *__restrict__ part,
-                                                    uint32_t *__restrict__ cpart) {
-    const uint32_t s = blockIdx.y;
-    const uint32_t kc = blockIdx.z;
-    const uint64_t col = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool colok = col < bpv;
-    const uint64_t sb = split_base + (uint64_t)s * rps;
-    const uint64_t r_begin = p0 > sb ? p0 : sb;
-    const uint64_t se = sb + rps;
-    const uint64_t r_end = p1 < se ? p1 : se;
-    const bool do_count = count_in_kernel && kc == 0;
-    const double *__restrict__ wbase = plan.w64 + (uint64_t)kc * KC;
+constexpr int kF32Group = 16;                       // rows per A operand (ABID 0..15)
+constexpr int kF32FlushGroups = 14;                 // 224-row FP32 blocks (even: groups go in pairs;
+                                                    // byte counters stay below 256)
+constexpr int kF32FlushRows = kF32Group * kF32FlushGroups;
+constexpr uint32_t kLutA2 = 0x02010000u;            // code 00,01,10,11 -> A2 dosage 0,0(missing),1,2
+constexpr uint32_t kLutA1 = 0x00010002u;            // the same codes read for the A1 allele: 2,0,1,0
+constexpr uint32_t kLutMiss = 0x00000100u;          // code 01 -> 1
+constexpr int nc1(int ncls) { return ncls > 0 ? ncls : 1; }
 
-    double acc[4][KC];
-    uint32_t cnt[4][kFp32MaxClasses];
+// One group of 16 program rows: the lane's byte of each row and the A
+// operands holding the weights of all 16 rows (vector registers), and per
+// row (bit q, scalar) whether it reads the A1 allele and whether it counts
+// missing calls for membership class 0.
+template <int KC>
+struct F32Group {
+    uint32_t b[kF32Group];
+    float w[KC / 4];
+    uint32_t flips, cnt0;
+    uint32_t g0;
+};
+
+template <int Q, int KC, int NCLS>
+__device__ __forceinline__ void fp32_row(const F32Group<KC> &g, const uint32_t *__restrict__ pcls, uint32_t nrows,
+                                         uint32_t vmask, f32x4_t (&acc)[4][KC / 4], uint32_t (&cnt4)[nc1(NCLS)]) {
+    const uint32_t b = g.b[Q];
+    uint32_t y = b | (b << 6);
+    y = y | (y << 12);
+    const uint32_t fsel = (uint32_t)__builtin_amdgcn_sbfe((int)g.flips, Q, 1) & 0x04040404u;
+    // the code mask sits in a vector register, so the and-or with the scalar
+    // flip selector is one instruction
+    const uint32_t sel = (y & vmask) | fsel;
+    const uint32_t d4 = __builtin_amdgcn_perm(kLutA1, kLutA2, sel);
+    const float d[4] = {(float)(d4 & 0xFFu), (float)((d4 >> 8) & 0xFFu), (float)((d4 >> 16) & 0xFFu),
+                        (float)(d4 >> 24)};
+#pragma unroll
+    for (int c = 0; c < KC / 4; ++c)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[t][c] = __builtin_amdgcn_mfma_f32_4x4x1f32(g.w[c], d[t], acc[t][c], 4, Q, 0);
+    if constexpr (NCLS == 1) {
+        // a selector of 0 picks byte 0 of the LUT, which is 0
+        const uint32_t cm = (uint32_t)__builtin_amdgcn_sbfe((int)g.cnt0, Q, 1);
+        cnt4[0] += __builtin_amdgcn_perm(kLutMiss, kLutMiss, sel & cm);
+    } else if constexpr (NCLS > 1) {
+        const uint32_t o = g.g0 + Q;
+        const uint32_t cls = o < nrows ? pcls[o] : 0u;
+        const uint32_t m4 = __builtin_amdgcn_perm(kLutMiss, kLutMiss, sel);
+#pragma unroll
+        for (int c = 0; c < NCLS; ++c) cnt4[c] += ((cls >> c) & 1u) ? m4 : 0u;
+    }
+}
+
+template <int KC, int NCLS, int... Qs>
+__device__ __forceinline__ void fp32_group(std::integer_sequence<int, Qs...>, const F32Group<KC> &g,
+                                           const uint32_t *__restrict__ pcls, uint32_t nrows, uint32_t vmask,
+                                           f32x4_t (&acc)[4][KC / 4], uint32_t (&cnt4)[nc1(NCLS)]) {
+    (fp32_row<Qs, KC, NCLS>(g, pcls, nrows, vmask, acc, cnt4), ...);
+}
+
+// Grid: one workgroup per (score pass, 1024-person block, variant split),
+// numbered so that the passes and person blocks of one split are consecutive
+// workgroups on one XCD (workgroup i runs on XCD i mod 8): the passes of a
+// row strip then read it through one L2.  Row positions inside a split are
+// 32-bit offsets from its first row (the scalar unit has no 64-bit ordered
+// compare).  IDENT: the block's program rows are its variants in order, so
+// row o of the split sits at rows + (base - p0 + o) * stride; otherwise the
+// wave loads 16 rows' variant indices into lanes and reads them back one by
+// one.
+template <int KC, int NCLS, bool IDENT>
+__global__ __launch_bounds__(256) void k_score_fp32(DevicePlan plan, const uint8_t *__restrict__ rows,
+                                                    uint64_t stride, uint64_t bpv, uint64_t v0, uint64_t p0,
+                                                    uint64_t p1, uint64_t rps, uint32_t npass, uint32_t npb,
+                                                    uint32_t nsplit, uint64_t num_people, uint64_t num_scores,
+                                                    double *__restrict__ part, uint32_t *__restrict__ cpart) {
+    const uint32_t total = npass * npb * nsplit;
+    const uint32_t per_xcd = (total + 7) / 8;
+    const uint32_t logical = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (logical >= total) return;
+    const uint32_t pass = logical % npass;
+    const uint32_t pb = (logical / npass) % npb;
+    const uint32_t s = logical / (npass * npb);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const uint32_t col = pb * 256u + (uint32_t)tid;
+    const bool colok = col < bpv;
+    // lanes past the row end read the last byte; their people are never written
+    const uint32_t lcol = colok ? col : (uint32_t)(bpv - 1);
+    const uint64_t sb = p0 + (uint64_t)s * rps;
+    const uint64_t base = sb < p1 ? sb : p1;
+    const uint32_t nrows = (uint32_t)(((sb + rps) < p1 ? (sb + rps) : p1) - base);
+    const uint64_t kw = plan.k32;
+    const uint64_t nfw = (plan.num_prog_rows + 31) / 32;   // flip words
+    const uint32_t *__restrict__ pvar = plan.prog_var + base;
+    const uint32_t *__restrict__ pcls = plan.prog_cls + base;
+    const float *__restrict__ wcol = plan.w32 + base * kw + (uint64_t)pass * KC + (lane & 3);
+    const uint8_t *__restrict__ rows0 = rows + (IDENT ? (base - p0) * stride : 0);
+
+    f32x4_t acc[4][KC / 4];
+    double accd[4][KC];
+    uint32_t cnt4[nc1(NCLS)], cntu[4][nc1(NCLS)];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
 #pragma unroll
-        for (int k = 0; k < KC; ++k) acc[t][k] = 0.0;
+        for (int k = 0; k < KC; ++k) accd[t][k] = 0.0;
 #pragma unroll
-        for (int c = 0; c < kFp32MaxClasses; ++c) cnt[t][c] = 0;
+        for (int c = 0; c < nc1(NCLS); ++c) cntu[t][c] = 0;
     }
-    // eight rows per step, their loads issued together and one step ahead
-    constexpr int kRB = 8;
-    uint32_t nb[kRB];
-    auto load_rows = [&](uint64_t r0, uint32_t *out) {
+    // Rows past the split read its last row with weight 0, no flip and no
+    // count.
+    auto load_group = [&](uint32_t g0, F32Group<KC> &g) {
+        const uint32_t ol = g0 + (uint32_t)(lane & 15);
+        const uint32_t olc = ol < nrows ? ol : nrows - 1;
+        if constexpr (IDENT) {
+            // one descriptor per group, the row step as the scalar offset
+            const uint32_t gc = g0 < nrows ? g0 : nrows - 1;
+            const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(rows0 + (uint64_t)gc * stride), 0, 0x7FFFFFFF, kRsrcWord3);
 #pragma unroll
-        for (int i = 0; i < kRB; ++i) {
-            const uint64_t r = r0 + i;
-            uint32_t b = 0u;  // rows past the split: code 00 unflipped, weight rows unused
-            if (r < r_end) {
-                const uint64_t var = (uint64_t)(plan.prog_var[r] - (uint32_t)v0);
-                b = colok ? rows[var * stride + col] : 0x55u;
-                if ((plan.flipmask[r >> 5] >> (r & 31)) & 1u) b = flip_byte(b);
+            for (int q = 0; q < kF32Group; ++q) {
+                const uint32_t o = g0 + q;
+                const uint32_t oc = o < nrows ? o : nrows - 1;
+                g.b[q] = __builtin_amdgcn_raw_buffer_load_b8(rr, lcol, (oc - gc) * (uint32_t)stride, 0);
             }
-            out[i] = b;
+        } else {
+            const uint32_t vl = pvar[olc] - (uint32_t)v0;
+#pragma unroll
+            for (int q = 0; q < kF32Group; ++q) {
+                const uint32_t var = (uint32_t)__builtin_amdgcn_readlane((int)vl, q);
+                const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
+                    (void *)(rows + (uint64_t)var * stride), 0, 0x7FFFFFFF, kRsrcWord3);
+                g.b[q] = __builtin_amdgcn_raw_buffer_load_b8(rr, lcol, 0, 0);
+            }
         }
+        const uint32_t wo = g0 + (uint32_t)(lane >> 2);
+        const uint32_t woc = wo < nrows ? wo : nrows - 1;
+#pragma unroll
+        for (int c = 0; c < KC / 4; ++c) {
+            const float w = wcol[(uint64_t)woc * kw + 4 * c];
+            g.w[c] = wo < nrows ? w : 0.0f;
+        }
+        const uint32_t left = g0 < nrows ? nrows - g0 : 0u;
+        const uint32_t in = left >= 32u ? 0xFFFFFFFFu : ((1u << left) - 1u);
+        const uint64_t r0 = base + (g0 < nrows ? g0 : nrows - 1);
+        const uint64_t wi = r0 >> 5;
+        const uint64_t lo = plan.flipmask[wi];
+        const uint64_t hi = wi + 1 < nfw ? plan.flipmask[wi + 1] : 0u;
+        g.flips = (uint32_t)(((hi << 32) | lo) >> (r0 & 31)) & in;
+        if constexpr (NCLS == 1) {
+            const bool c0 = (lane < 16) && (ol < nrows) && (pass == 0) && (pcls[olc] & 1u);
+            g.cnt0 = (uint32_t)__builtin_amdgcn_ballot_w64(c0);
+        }
+        g.g0 = g0;
     };
-    if (r_begin < r_end) load_rows(r_begin, nb);
-    for (uint64_t r0 = r_begin; r0 < r_end; r0 += kRB) {
-        uint32_t cb[kRB];
+
+    uint32_t vmask;
+    asm volatile("v_mov_b32 %0, 0x3030303" : "=v"(vmask));
+    F32Group<KC> ga, gb;
+    if (nrows) load_group(0, ga);
+    // Outer loop over FP32 blocks, inner loop over pairs of groups (the two
+    // group buffers alternate, one loading while the other is multiplied;
+    // a pair reaching past the block's rows multiplies zero weights).  The
+    // FP64 flush sits after the inner loop, so the accumulators stay in
+    // accumulation registers inside it.
+    for (uint32_t blk0 = 0; blk0 < nrows; blk0 += kF32FlushRows) {
 #pragma unroll
-        for (int i = 0; i < kRB; ++i) cb[i] = nb[i];
-        if (r0 + kRB < r_end) load_rows(r0 + kRB, nb);
+        for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int i = 0; i < kRB; ++i) {
-            const uint64_t r = r0 + i;
-            if (r >= r_end) break;
-            const uint32_t b = cb[i];
-            const double *__restrict__ w = wbase + r * plan.k32;
-            double wk[KC];
+            for (int c = 0; c < KC / 4; ++c) acc[t][c] = f32x4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int k = 0; k < KC; ++k) wk[k] = w[k];
+        for (int c = 0; c < nc1(NCLS); ++c) cnt4[c] = 0u;
+        const uint32_t blk1 = nrows - blk0 > (uint32_t)kF32FlushRows ? blk0 + kF32FlushRows : nrows;
+        for (uint32_t g0 = blk0; g0 < blk1; g0 += 2 * kF32Group) {
+            load_group(g0 + kF32Group, gb);
+            fp32_group<KC, NCLS>(std::make_integer_sequence<int, kF32Group>{}, ga, pcls, nrows, vmask, acc, cnt4);
+            load_group(g0 + 2 * kF32Group, ga);
+            fp32_group<KC, NCLS>(std::make_integer_sequence<int, kF32Group>{}, gb, pcls, nrows, vmask, acc, cnt4);
+        }
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const uint32_t c = (b >> (2 * t)) & 3u;
-                const double d = (double)((c >> 1) * ((c & 1u) + 1u));
+        for (int t = 0; t < 4; ++t) {
 #pragma unroll
-                for (int k = 0; k < KC; ++k) acc[t][k] = __builtin_fma(d, wk[k], acc[t][k]);
-            }
-            if (do_count) {
-                const uint32_t cls = plan.prog_cls[r];
-                if (cls) {
+            for (int c = 0; c < KC / 4; ++c)
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        const uint32_t m = (((b >> (2 * t)) & 3u) == 1u) ? 1u : 0u;
+                for (int i = 0; i < 4; ++i) accd[t][4 * c + i] += (double)acc[t][c][i];
 #pragma unroll
-                        for (int c = 0; c < kFp32MaxClasses; ++c) cnt[t][c] += m & (cls >> c);
-                    }
-                }
-            }
+            for (int c = 0; c < nc1(NCLS); ++c) cntu[t][c] += (cnt4[c] >> (8 * t)) & 0xFFu;
         }
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        const uint64_t p = col * 4 + t;
-        if (p >= num_people) continue;
-        double *dst = part + ((uint64_t)s * num_people + p) * num_scores + (uint64_t)kc * KC;
+        const uint64_t p = (uint64_t)col * 4 + t;
+        if (!colok || p >= num_people) continue;
+        double *dst = part + ((uint64_t)s * num_people + p) * num_scores + (uint64_t)pass * KC;
 #pragma unroll
         for (int k = 0; k < KC; ++k)
-            if ((uint64_t)kc * KC + k < num_scores) dst[k] = acc[t][k];
-        if (do_count) {
+            if ((uint64_t)pass * KC + k < num_scores) dst[k] = accd[t][k];
+        if (NCLS > 0 && pass == 0) {
             uint32_t *cd = cpart + ((uint64_t)s * num_people + p) * plan.nclass;
 #pragma unroll
-            for (int c = 0; c < kFp32MaxClasses; ++c)
-                if ((uint32_t)c < plan.nclass) cd[c] = cnt[t][c];
+            for (int c = 0; c < nc1(NCLS); ++c)
+                if ((uint32_t)c < plan.nclass) cd[c] = cntu[t][c];
         }
     }
+}
+
+// scores per pass of the FP32 kernel for num_scores scores
+static uint32_t fp32_pass_scores(uint64_t num_scores) { return num_scores <= 4 ? 4u : 8u; }
+
+uint64_t fp32_weight_pitch(uint64_t num_scores) {
+    const uint64_t kc = fp32_pass_scores(num_scores);
+    return (num_scores + kc - 1) / kc * kc;
 }
 
 hipError_t launch_score_fp32(const DevicePlan &plan, const uint8_t *rows, uint64_t stride,
                              uint64_t bpv, uint64_t v0, uint64_t p0, uint64_t p1,
                              uint64_t num_people, uint64_t num_scores, uint32_t splits,
-                             bool count_in_kernel, double *part, uint32_t *cpart, hipStream_t stream) {
+                             bool count_in_kernel, bool identity, double *part, uint32_t *cpart,
+                             hipStream_t stream) {
     const uint64_t cols = (num_people + 3) / 4;
-    const uint64_t split_base = p0;
-    const uint64_t rows_per_split = (p1 - p0 + splits - 1) / splits;
-    dim3 grid((unsigned)((cols + 255) / 256), splits,
-              (unsigned)((num_scores + kFp32ScoreChunk - 1) / kFp32ScoreChunk));
-    hipLaunchKernelGGL(k_score_fp32<kFp32ScoreChunk>, grid, dim3(256), 0, stream, plan, rows,
-                       stride, bpv, v0, p0, p1, split_base, rows_per_split, num_people,
-                       num_scores, count_in_kernel ? 1 : 0, part, cpart);
+    const uint32_t kc = fp32_pass_scores(num_scores);
+    if (plan.k32 != fp32_weight_pitch(num_scores)) return hipErrorInvalidValue;
+    const uint32_t npass = (uint32_t)((num_scores + kc - 1) / kc);
+    const uint32_t npb = (uint32_t)((cols + 255) / 256);
+    const uint64_t rps = (p1 - p0 + splits - 1) / splits;
+    const uint64_t total = (uint64_t)npass * npb * splits;
+    // the row step of a 16-row group must fit the 32-bit scalar offset of a buffer load
+    if (total >= (1ull << 31) || bpv == 0 || bpv > 0x7FFFFFFFull || stride > (0x7FFFFFFFull / kF32Group) ||
+        rps >= (1ull << 32) || p1 < p0)
+        return hipErrorInvalidValue;
+    const int ncls = count_in_kernel ? (plan.nclass <= 1 ? 1 : kFp32MaxClasses) : 0;
+    const dim3 grid((unsigned)((total + 7) / 8 * 8)), block(256);
+#define GNM_F32(KC, NC, ID)                                                                                        \
+    hipLaunchKernelGGL((k_score_fp32<KC, NC, ID>), grid, block, 0, stream, plan, rows, stride, bpv, v0, p0, p1,    \
+                       rps, npass, npb, splits, num_people, num_scores, part, cpart)
+#define GNM_F32_ID(KC, NC)                                                                                         \
+    do {                                                                                                           \
+        if (identity) GNM_F32(KC, NC, true);                                                                       \
+        else GNM_F32(KC, NC, false);                                                                               \
+    } while (0)
+    if (kc == 4) {
+        if (ncls == 1) GNM_F32_ID(4, 1);
+        else if (ncls == 0) GNM_F32_ID(4, 0);
+        else GNM_F32_ID(4, kFp32MaxClasses);
+    } else {
+        if (ncls == 1) GNM_F32_ID(8, 1);
+        else if (ncls == 0) GNM_F32_ID(8, 0);
+        else GNM_F32_ID(8, kFp32MaxClasses);
+    }
+#undef GNM_F32_ID
+#undef GNM_F32
     return hipGetLastError();
 }
 
@@ -498,8 +653,8 @@ static uint32_t splits_for(uint64_t wgs_per_split) {
 
 uint32_t fp32_splits(uint64_t num_people, uint64_t num_scores) {
     const uint64_t pt = ((num_people + 3) / 4 + 255) / 256;
-    const uint64_t chunks = (num_scores + kFp32ScoreChunk - 1) / kFp32ScoreChunk;
-    return splits_for(pt * chunks);
+    const uint64_t passes = (num_scores + fp32_pass_scores(num_scores) - 1) / fp32_pass_scores(num_scores);
+    return splits_for(pt * passes);
 }
 
 // ---------------------------------------------------------------------------

@@ -14,7 +14,7 @@
 //                   max |w| <= qmax * 2^e, and the limb scales;
 //   k_plan_frag_i8  split-int8 B fragments, [k-step][tile][lane][16 i8];
 //   k_plan_frag_bf16 split-BF16 B fragments, [stage][tile][k-step][lane][8 bf16];
-//   k_plan_w64      FP32-path weights (W rounded once to f32, held as f64).
+//   k_plan_w32      FP32-path weights (W rounded once to f32).
 #include "gnm_internal.hpp"
 
 #include <cmath>
@@ -149,12 +149,12 @@ __global__ __launch_bounds__(256) void k_plan_frag_bf16(const double *__restrict
     frag[idx] = make_uint4(word[0], word[1], word[2], word[3]);
 }
 
-__global__ __launch_bounds__(256) void k_plan_w64(const double *__restrict__ wrow, uint64_t R, uint64_t K, uint64_t k32,
-                                                  double *__restrict__ w64) {
+__global__ __launch_bounds__(256) void k_plan_w32(const double *__restrict__ wrow, uint64_t R, uint64_t K, uint64_t k32,
+                                                  float *__restrict__ w32) {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= R * K) return;
     const uint64_t r = i / K, k = i % K;
-    w64[r * k32 + k] = (double)(float)wrow[i];
+    w32[r * k32 + k] = (float)wrow[i];
 }
 
 static unsigned blocks_for(uint64_t n) { return (unsigned)((n + 255) / 256); }
@@ -194,9 +194,9 @@ hipError_t launch_plan_frag(bool int8, const double *wrow, uint64_t R, uint64_t 
     return hipGetLastError();
 }
 
-hipError_t launch_plan_w64(const double *wrow, uint64_t R, uint64_t K, uint64_t k32, double *w64, hipStream_t stream) {
+hipError_t launch_plan_w32(const double *wrow, uint64_t R, uint64_t K, uint64_t k32, float *w32, hipStream_t stream) {
     if (R * K == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_plan_w64, dim3(blocks_for(R * K)), dim3(256), 0, stream, wrow, R, K, k32, w64);
+    hipLaunchKernelGGL(k_plan_w32, dim3(blocks_for(R * K)), dim3(256), 0, stream, wrow, R, K, k32, w32);
     return hipGetLastError();
 }
 

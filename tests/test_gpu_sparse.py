@@ -14,7 +14,7 @@ torch = pytest.importorskip("torch")
 from gnomon_amd._lib import GnmError
 from gnomon_amd.pipeline import ScoringContext
 from gnomon_amd.prep import csr_from_dense
-from tests.helpers import normwise_rel, reference_agree
+from tests.helpers import abs_terms, avg_agree, fp32_sum_bound, normwise_rel, reference_agree
 
 pytestmark = pytest.mark.gpu
 TOL = {"fp32": None, "bf16x3": 1e-6, "i8x4": 1e-6}
@@ -24,8 +24,15 @@ def _flush(msg):
     print(msg, flush=True)
 
 
-def _check(S, C, S64, C64, precision):
+def _check(S, C, S64, C64, precision, fp32=None):
+    """fp32: (orc, rows, csr, N, out_to_fam) -- the FP32 path is held to the
+    FP32 summation bound and the reference's rule on AVG values."""
     assert np.array_equal(C, C64), "missing counts differ"
+    if precision == "fp32":
+        orc, rows, csr, n, o2f = fp32
+        assert np.all(np.abs(S - S64) <= fp32_sum_bound(abs_terms(orc, rows, csr, n, o2f)))
+        assert avg_agree(S, S64, C, csr, 1e-6).all()
+        return
     ok = reference_agree(S, S64, 1e-6)
     assert ok.all(), f"{(~ok).sum()} cells outside the 1e-6 rule; worst {np.abs(S - S64).max()}"
     if TOL[precision]:
@@ -65,7 +72,7 @@ def test_sparse_ingest_bit_identical_to_packed(orc, precision):
     S, C, st = _run(csr, rows, N, precision, 0.5, blocks=3)
     assert st["sparse_rows"] > V // 4 and st["dense_rows"] > 0, st
     assert st["bytes_to_device"] < V * rows.shape[1]
-    _check(S, C, S64, C64, precision)
+    _check(S, C, S64, C64, precision, (orc, rows, csr, N, None))
     Sd, Cd, std = _run(csr, rows, N, precision, 0.0, blocks=3)
     assert std["sparse_rows"] == 0 and std["dense_rows"] == V
     assert np.array_equal(Sd, S) and np.array_equal(Cd, C)
@@ -95,7 +102,7 @@ def test_sparse_ingest_person_subset(orc):
     for precision in ("fp32", "i8x4"):
         S, C, st = _run(csr, rows, 9000, precision, 0.5, nfam=Nfam, o2f=o2f, blocks=2)
         assert st["sparse_rows"] > 0
-        _check(S, C, S64, C64, precision)
+        _check(S, C, S64, C64, precision, (orc, rows, csr, 9000, o2f))
         Sd, Cd, _ = _run(csr, rows, 9000, precision, 0.0, nfam=Nfam, o2f=o2f, blocks=2)
         assert np.array_equal(Sd, S) and np.array_equal(Cd, C)
 
