@@ -394,61 +394,59 @@ hipError_t launch_compact(const uint8_t *rows, uint64_t stride, uint64_t num_row
 // Missing calls are counted as packed bytes, at most kF32FlushRows per block.
 // ---------------------------------------------------------------------------
 constexpr int kF32Group = 16;                       // rows per A operand (ABID 0..15)
-constexpr int kF32FlushGroups = 14;                 // 224-row FP32 blocks (even: groups go in pairs;
-                                                    // byte counters stay below 256)
+constexpr int kF32FlushGroups = 14;                 // 224-row FP32 blocks (byte counters stay below 256)
 constexpr int kF32FlushRows = kF32Group * kF32FlushGroups;
 constexpr uint32_t kLutA2 = 0x02010000u;            // code 00,01,10,11 -> A2 dosage 0,0(missing),1,2
 constexpr uint32_t kLutA1 = 0x00010002u;            // the same codes read for the A1 allele: 2,0,1,0
 constexpr uint32_t kLutMiss = 0x00000100u;          // code 01 -> 1
 constexpr int nc1(int ncls) { return ncls > 0 ? ncls : 1; }
 
-// One group of 16 program rows: the lane's byte of each row and the A
-// operands holding the weights of all 16 rows (vector registers), and per
-// row (bit q, scalar) whether it reads the A1 allele and whether it counts
-// missing calls for membership class 0.
+// Where the rows of a group of 16 program rows come from, and their weights
+// and flags as loaded.  A group is multiplied one group after its loads
+// were issued; row q's byte of the next group is loaded into the register
+// of row q as soon as row q has been multiplied (one byte register per row,
+// a 16-row prefetch distance).
 template <int KC>
-struct F32Group {
-    uint32_t b[kF32Group];
-    float w[KC / 4];
-    uint32_t flips, cnt0;
-    uint32_t g0;
+struct F32Src {
+    uint32_t g0;                  // first row (offset in the split)
+    __amdgpu_buffer_rsrc_t rr;    // IDENT: the group's first row (clamped)
+    uint32_t last;                // IDENT: byte offset from it of the group's last row inside the split
+    uint32_t vl;                  // not IDENT: lane q < 16 holds row q's block row index
+    float w[KC / 4];              // lane 4q + i: score 4c + i of row q (rows past the split masked later)
+    uint32_t fw0, fw1, fsh;       // flip words holding the group's rows, bit of row g0
+    uint32_t pc;                  // lane q < 16: class bits of row q
 };
 
 template <int Q, int KC, int NCLS>
-__device__ __forceinline__ void fp32_row(const F32Group<KC> &g, const uint32_t *__restrict__ pcls, uint32_t nrows,
+__device__ __forceinline__ void fp32_row(uint32_t b, uint32_t flips, uint32_t cnt0, const float (&w)[KC / 4],
+                                         uint32_t g0, const uint32_t *__restrict__ pcls, uint32_t nrows,
                                          uint32_t vmask, f32x4_t (&acc)[4][KC / 4], uint32_t (&cnt4)[nc1(NCLS)]) {
-    const uint32_t b = g.b[Q];
     uint32_t y = b | (b << 6);
     y = y | (y << 12);
-    const uint32_t fsel = (uint32_t)__builtin_amdgcn_sbfe((int)g.flips, Q, 1) & 0x04040404u;
-    // the code mask sits in a vector register, so the and-or with the scalar
-    // flip selector is one instruction
-    const uint32_t sel = (y & vmask) | fsel;
+    const uint32_t fsel = (uint32_t)__builtin_amdgcn_sbfe((int)flips, Q, 1) & 0x04040404u;
+    // (y & vmask) | fsel in one v_bitop3 (truth table 0xEA over y, vmask,
+    // fsel); the code mask sits in a vector register, so the scalar flip
+    // selector is the instruction's one scalar operand
+    const uint32_t sel = __builtin_amdgcn_bitop3_b32(y, vmask, fsel, 0xEA);
     const uint32_t d4 = __builtin_amdgcn_perm(kLutA1, kLutA2, sel);
     const float d[4] = {(float)(d4 & 0xFFu), (float)((d4 >> 8) & 0xFFu), (float)((d4 >> 16) & 0xFFu),
                         (float)(d4 >> 24)};
 #pragma unroll
     for (int c = 0; c < KC / 4; ++c)
 #pragma unroll
-        for (int t = 0; t < 4; ++t) acc[t][c] = __builtin_amdgcn_mfma_f32_4x4x1f32(g.w[c], d[t], acc[t][c], 4, Q, 0);
+        for (int t = 0; t < 4; ++t) acc[t][c] = __builtin_amdgcn_mfma_f32_4x4x1f32(w[c], d[t], acc[t][c], 4, Q, 0);
     if constexpr (NCLS == 1) {
-        // a selector of 0 picks byte 0 of the LUT, which is 0
-        const uint32_t cm = (uint32_t)__builtin_amdgcn_sbfe((int)g.cnt0, Q, 1);
-        cnt4[0] += __builtin_amdgcn_perm(kLutMiss, kLutMiss, sel & cm);
+        // y & vmask & cm (truth table 0x80): the codes of a count row, all
+        // zero otherwise -- a selector of 0 picks byte 0 of the LUT, which is 0
+        const uint32_t cm = (uint32_t)__builtin_amdgcn_sbfe((int)cnt0, Q, 1);
+        cnt4[0] += __builtin_amdgcn_perm(kLutMiss, kLutMiss, __builtin_amdgcn_bitop3_b32(y, vmask, cm, 0x80));
     } else if constexpr (NCLS > 1) {
-        const uint32_t o = g.g0 + Q;
+        const uint32_t o = g0 + Q;
         const uint32_t cls = o < nrows ? pcls[o] : 0u;
         const uint32_t m4 = __builtin_amdgcn_perm(kLutMiss, kLutMiss, sel);
 #pragma unroll
         for (int c = 0; c < NCLS; ++c) cnt4[c] += ((cls >> c) & 1u) ? m4 : 0u;
     }
-}
-
-template <int KC, int NCLS, int... Qs>
-__device__ __forceinline__ void fp32_group(std::integer_sequence<int, Qs...>, const F32Group<KC> &g,
-                                           const uint32_t *__restrict__ pcls, uint32_t nrows, uint32_t vmask,
-                                           f32x4_t (&acc)[4][KC / 4], uint32_t (&cnt4)[nc1(NCLS)]) {
-    (fp32_row<Qs, KC, NCLS>(g, pcls, nrows, vmask, acc, cnt4), ...);
 }
 
 // Grid: one workgroup per (score pass, 1024-person block, variant split),
@@ -459,7 +457,8 @@ __device__ __forceinline__ void fp32_group(std::integer_sequence<int, Qs...>, co
 // compare).  IDENT: the block's program rows are its variants in order, so
 // row o of the split sits at rows + (base - p0 + o) * stride; otherwise the
 // wave loads 16 rows' variant indices into lanes and reads them back one by
-// one.
+// one.  Rows past the split read its last row; they are multiplied with
+// weight 0, no flip and no count.
 template <int KC, int NCLS, bool IDENT>
 __global__ __launch_bounds__(256) void k_score_fp32(DevicePlan plan, const uint8_t *__restrict__ rows,
                                                     uint64_t stride, uint64_t bpv, uint64_t v0, uint64_t p0,
@@ -485,8 +484,11 @@ __global__ __launch_bounds__(256) void k_score_fp32(DevicePlan plan, const uint8
     const uint64_t nfw = (plan.num_prog_rows + 31) / 32;   // flip words
     const uint32_t *__restrict__ pvar = plan.prog_var + base;
     const uint32_t *__restrict__ pcls = plan.prog_cls + base;
+    const uint32_t *__restrict__ fmask = plan.flipmask;
     const float *__restrict__ wcol = plan.w32 + base * kw + (uint64_t)pass * KC + (lane & 3);
     const uint8_t *__restrict__ rows0 = rows + (IDENT ? (base - p0) * stride : 0);
+    // the code mask as a per-lane value (never 1: col stays far below 2^32 - 16)
+    const uint32_t vmask = 0x03030303u + (uint32_t)(col > 0xFFFFFFF0u);
 
     f32x4_t acc[4][KC / 4];
     double accd[4][KC];
@@ -498,62 +500,73 @@ __global__ __launch_bounds__(256) void k_score_fp32(DevicePlan plan, const uint8
 #pragma unroll
         for (int c = 0; c < nc1(NCLS); ++c) cntu[t][c] = 0;
     }
-    // Rows past the split read its last row with weight 0, no flip and no
-    // count.
-    auto load_group = [&](uint32_t g0, F32Group<KC> &g) {
+
+    // everything of group g0 but its row bytes
+    auto source = [&](uint32_t g0, F32Src<KC> &x) {
+        x.g0 = g0;
         const uint32_t ol = g0 + (uint32_t)(lane & 15);
         const uint32_t olc = ol < nrows ? ol : nrows - 1;
         if constexpr (IDENT) {
-            // one descriptor per group, the row step as the scalar offset
             const uint32_t gc = g0 < nrows ? g0 : nrows - 1;
-            const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
-                (void *)(rows0 + (uint64_t)gc * stride), 0, 0x7FFFFFFF, kRsrcWord3);
-#pragma unroll
-            for (int q = 0; q < kF32Group; ++q) {
-                const uint32_t o = g0 + q;
-                const uint32_t oc = o < nrows ? o : nrows - 1;
-                g.b[q] = __builtin_amdgcn_raw_buffer_load_b8(rr, lcol, (oc - gc) * (uint32_t)stride, 0);
-            }
+            const uint32_t gl = nrows - 1 < g0 + (kF32Group - 1) ? nrows - 1 : g0 + (kF32Group - 1);
+            x.rr = __builtin_amdgcn_make_buffer_rsrc((void *)(rows0 + (uint64_t)gc * stride), 0, 0x7FFFFFFF,
+                                                     kRsrcWord3);
+            x.last = (gl - gc) * (uint32_t)stride;
         } else {
-            const uint32_t vl = pvar[olc] - (uint32_t)v0;
-#pragma unroll
-            for (int q = 0; q < kF32Group; ++q) {
-                const uint32_t var = (uint32_t)__builtin_amdgcn_readlane((int)vl, q);
-                const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
-                    (void *)(rows + (uint64_t)var * stride), 0, 0x7FFFFFFF, kRsrcWord3);
-                g.b[q] = __builtin_amdgcn_raw_buffer_load_b8(rr, lcol, 0, 0);
-            }
+            x.vl = pvar[olc] - (uint32_t)v0;
         }
         const uint32_t wo = g0 + (uint32_t)(lane >> 2);
         const uint32_t woc = wo < nrows ? wo : nrows - 1;
 #pragma unroll
-        for (int c = 0; c < KC / 4; ++c) {
-            const float w = wcol[(uint64_t)woc * kw + 4 * c];
-            g.w[c] = wo < nrows ? w : 0.0f;
-        }
-        const uint32_t left = g0 < nrows ? nrows - g0 : 0u;
-        const uint32_t in = left >= 32u ? 0xFFFFFFFFu : ((1u << left) - 1u);
+        for (int c = 0; c < KC / 4; ++c) x.w[c] = wcol[(uint64_t)woc * kw + 4 * c];
         const uint64_t r0 = base + (g0 < nrows ? g0 : nrows - 1);
         const uint64_t wi = r0 >> 5;
-        const uint64_t lo = plan.flipmask[wi];
-        const uint64_t hi = wi + 1 < nfw ? plan.flipmask[wi + 1] : 0u;
-        g.flips = (uint32_t)(((hi << 32) | lo) >> (r0 & 31)) & in;
-        if constexpr (NCLS == 1) {
-            const bool c0 = (lane < 16) && (ol < nrows) && (pass == 0) && (pcls[olc] & 1u);
-            g.cnt0 = (uint32_t)__builtin_amdgcn_ballot_w64(c0);
+        x.fw0 = fmask[wi];
+        x.fw1 = fmask[wi + 1 < nfw ? wi + 1 : wi];
+        x.fsh = (uint32_t)(r0 & 31);
+        if constexpr (NCLS == 1) x.pc = pass == 0 ? pcls[olc] : 0u;
+    };
+    auto load_byte = [&](const F32Src<KC> &x, int q) -> uint32_t {
+        uint32_t v;
+        if constexpr (IDENT) {
+            const uint32_t so = (uint32_t)q * (uint32_t)stride;
+            v = __builtin_amdgcn_raw_buffer_load_b8(x.rr, lcol, so < x.last ? so : x.last, 0);
+        } else {
+            const uint32_t var = (uint32_t)__builtin_amdgcn_readlane((int)x.vl, q);
+            const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(rows + (uint64_t)var * stride), 0, 0x7FFFFFFF, kRsrcWord3);
+            v = __builtin_amdgcn_raw_buffer_load_b8(rr, lcol, 0, 0);
         }
-        g.g0 = g0;
+        __builtin_assume(v <= 0xFFu);
+        return v;
+    };
+    // multiply group cur (bytes in b), loading the next group's rows into b
+    auto run_group = [&]<int... Qs>(std::integer_sequence<int, Qs...>, const F32Src<KC> &cur,
+                                    const F32Src<KC> &nxt, uint32_t (&b)[kF32Group]) {
+        const uint32_t left = cur.g0 < nrows ? nrows - cur.g0 : 0u;
+        const uint32_t in = left >= 32u ? 0xFFFFFFFFu : ((1u << left) - 1u);
+        const uint32_t flips = (uint32_t)((((uint64_t)cur.fw1 << 32) | cur.fw0) >> cur.fsh) & in;
+        uint32_t cnt0 = 0;
+        if constexpr (NCLS == 1)
+            cnt0 = (uint32_t)__builtin_amdgcn_ballot_w64(lane < 16 && ((in >> (lane & 15)) & 1u) && (cur.pc & 1u));
+        const bool wrow_in = (in >> (lane >> 2)) & 1u;   // lane 4q + i carries row q's weights
+        float w[KC / 4];
+#pragma unroll
+        for (int c = 0; c < KC / 4; ++c) w[c] = wrow_in ? cur.w[c] : 0.0f;
+        ((fp32_row<Qs, KC, NCLS>(b[Qs], flips, cnt0, w, cur.g0, pcls, nrows, vmask, acc, cnt4),
+          b[Qs] = load_byte(nxt, Qs)), ...);
     };
 
-    uint32_t vmask;
-    asm volatile("v_mov_b32 %0, 0x3030303" : "=v"(vmask));
-    F32Group<KC> ga, gb;
-    if (nrows) load_group(0, ga);
-    // Outer loop over FP32 blocks, inner loop over pairs of groups (the two
-    // group buffers alternate, one loading while the other is multiplied;
-    // a pair reaching past the block's rows multiplies zero weights).  The
-    // FP64 flush sits after the inner loop, so the accumulators stay in
-    // accumulation registers inside it.
+    F32Src<KC> sa, sn;
+    uint32_t b[kF32Group];
+    if (nrows) {
+        source(0, sa);
+#pragma unroll
+        for (int q = 0; q < kF32Group; ++q) b[q] = load_byte(sa, q);
+    }
+    // outer loop over FP32 blocks, inner loop over their groups: the FP64
+    // flush sits after the inner loop, so the accumulators stay in
+    // accumulation registers inside it
     for (uint32_t blk0 = 0; blk0 < nrows; blk0 += kF32FlushRows) {
 #pragma unroll
         for (int t = 0; t < 4; ++t)
@@ -562,11 +575,10 @@ __global__ __launch_bounds__(256) void k_score_fp32(DevicePlan plan, const uint8
 #pragma unroll
         for (int c = 0; c < nc1(NCLS); ++c) cnt4[c] = 0u;
         const uint32_t blk1 = nrows - blk0 > (uint32_t)kF32FlushRows ? blk0 + kF32FlushRows : nrows;
-        for (uint32_t g0 = blk0; g0 < blk1; g0 += 2 * kF32Group) {
-            load_group(g0 + kF32Group, gb);
-            fp32_group<KC, NCLS>(std::make_integer_sequence<int, kF32Group>{}, ga, pcls, nrows, vmask, acc, cnt4);
-            load_group(g0 + 2 * kF32Group, ga);
-            fp32_group<KC, NCLS>(std::make_integer_sequence<int, kF32Group>{}, gb, pcls, nrows, vmask, acc, cnt4);
+        for (uint32_t g0 = blk0; g0 < blk1; g0 += kF32Group) {
+            source(g0 + kF32Group, sn);
+            run_group(std::make_integer_sequence<int, kF32Group>{}, sa, sn, b);
+            sa = sn;
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
