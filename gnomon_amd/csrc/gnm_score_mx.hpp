@@ -167,13 +167,21 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(BF ?
     constexpr int kBU = kBT * KS;                    // B fragments per stage
     constexpr int kBW = (kBU + WAVES - 1) / WAVES;   // B fragments each wave fetches per stage
     constexpr uint64_t kFlush = BF ? (uint64_t)kBfFlushRows : (uint64_t)kI8FlushRows;
-    static_assert(kFlush % (D * kI8Stage) == 0, "scale blocks must hold whole batches");
+    static_assert(kFlush % (2 * D * kI8Stage) == 0, "scale blocks must hold whole pairs of batches");
     // one LDS tile per stage of a batch: all D stages are stored, then one
     // barrier, then all D are consumed (two barriers per batch of D stages)
     __shared__ uint32_t tile[D][kTileWords];
-    // B fragments are double-buffered: the next batch's DMA lands in one
-    // half while this batch's MFMAs read the other
-    __shared__ u32x4_t btile[2][D][kBU * 64];
+    // B fragments are double-buffered in two LDS objects: the next batch's
+    // DMA lands in one while this batch's MFMAs read the other.  Batches
+    // alternate between them with a compile-time parity, so the compiler
+    // sees that the reads never touch the object a DMA is writing and adds
+    // no memory-counter wait before them.
+    __shared__ u32x4_t btile0[D][kBU * 64];
+    __shared__ u32x4_t btile1[D][kBU * 64];
+    auto btile = [&](auto pc) __attribute__((always_inline)) -> u32x4_t(&)[D][kBU * 64] {
+        if constexpr (decltype(pc)::value == 0) return btile0;
+        else return btile1;
+    };
 
     // Workgroups are dispatched round-robin over the 8 XCDs (L mod 8).  One
     // launch covers every tile group: the groups of one (split, person tile)
@@ -241,12 +249,13 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(BF ?
 
     const uint64_t last_st = ((r_end - 1) / kI8Stage) * kI8Stage;
     // Stages are numbered from st0, the split's first row aligned down to a
-    // batch; per-stage tests use these 32-bit indices (scalar compares, where
+    // pair of batches (scale blocks start at even batch indices, so a block
+    // is a whole number of batch pairs); per-stage tests use these 32-bit indices (scalar compares, where
     // 64-bit row compares take vector instructions):
     //   nst      stages of the split
     //   fast_lo, fast_hi   stages wholly inside [r_begin, r_end) (fast path)
     //   last_i   the stage holding the split's last row
-    const uint64_t st0 = (r_begin / (D * kI8Stage)) * (D * kI8Stage);
+    const uint64_t st0 = (r_begin / (2 * D * kI8Stage)) * (2 * D * kI8Stage);
     const bool nonempty = r_begin < r_end;
     const uint32_t nst = nonempty ? (uint32_t)((r_end - st0 + kI8Stage - 1) / kI8Stage) : 0u;
     const uint32_t fast_lo = (uint32_t)((r_begin - st0 + kI8Stage - 1) / kI8Stage);
@@ -254,7 +263,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(BF ?
     const uint32_t last_i = nonempty ? (uint32_t)((last_st - st0) / kI8Stage) : 0u;
     // B fragments: wave w fetches fragments u = w, w + WAVES, ... of the
     // sets' kBU fragments (tile u / KS, k-step u % KS) by LDS DMA (lane l's
-    // 16 bytes land at btile[d][64 u + l]); tiles past tile_end are fetched
+    // 16 bytes land at btile(par)[d][64 u + l]); tiles past tile_end are fetched
     // clamped to the last tile and their products are never flushed
     uint32_t boff[kBW];     // byte offsets within a stage's fragment block
 #pragma unroll
@@ -267,14 +276,14 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(BF ?
     // fragment block of stage index si (clamped to last_i by the caller)
     const u32x4_t *__restrict__ bfrag_st0 = bfrag + (st0 / kI8Stage) * tiles_total * KS * 64;
     const uint32_t bstep = tiles_total * KS * 64;
-    auto fetch_b = [&](uint32_t par, int d, uint32_t si) __attribute__((always_inline)) {
+    auto fetch_b = [&](auto par, int d, uint32_t si) __attribute__((always_inline)) {
         const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
             (void *)(bfrag_st0 + (uint64_t)si * bstep), 0, 0x7FFFFFFF, kI8RsrcWord3);
 #pragma unroll
         for (int m = 0; m < kBW; ++m) {
             const int u = wave + m * WAVES;
             if (kBW * WAVES == kBU || u < kBU)
-                lds_dma_b128(rb, &btile[par][d][u * 64], boff[m]);
+                lds_dma_b128(rb, &btile(par)[d][u * 64], boff[m]);
         }
     };
     auto load_row = [&](uint64_t var, uint64_t lcol) __attribute__((always_inline)) -> uint32_t {
@@ -289,7 +298,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(BF ?
         }
         return x;
     };
-    auto issue = [&](uint32_t par, int d, uint32_t si) __attribute__((always_inline)) {
+    auto issue = [&](int d, uint32_t si) __attribute__((always_inline)) {
         const uint64_t st_req = st0 + (uint64_t)si * kI8Stage;
         if (si >= fast_lo && si < fast_hi) {   // IDENT && span_full, stage inside the split
             // buffer loads: a uniform descriptor per stage, one 32-bit lane
@@ -426,21 +435,35 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(BF ?
             const uint32_t nc = wtile ? ncol : nclass;
             if ((uint32_t)j < ntile && c < nc) {
                 const double sc = (wtile && !empty) ? plan.scales[blk * ncol + c] : 0.0;
+                // 16 people per lane in two halves of 8: a half's earlier
+                // partials are all loaded before any is updated, so a
+                // read-modify-write flush waits for one load latency per
+                // half instead of one per person
 #pragma unroll 1
-                for (int i = 0; i < 16; ++i) {
-                    const uint32_t pl = 4 * i + lg;  // person of the wave, 0..63
-                    if (pw0 + pl >= num_people) continue;
-                    const int vb = sbuf[pl * 16 + lane_c];
-                    const double v = BF ? (double)__builtin_bit_cast(float, vb) : (double)vb;
-                    const uint64_t row = prow + pl;
-                    if (wtile) {
-                        double *dd = part + row * ncol + c;
-                        const double x = v * sc;
-                        *dd = first ? x : *dd + x;
-                    } else {
-                        uint32_t *dd = cpart + row * nclass + c;
-                        const uint32_t x = (uint32_t)v;
-                        *dd = first ? x : *dd + x;
+                for (int h = 0; h < 2; ++h) {
+                    double oldw[8];
+                    uint32_t oldc[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        const uint32_t pl = 4 * (8 * h + i) + lg;   // person of the wave, 0..63
+                        const bool ok = pw0 + pl < num_people && !first;
+                        const uint64_t row = prow + pl;
+                        oldw[i] = 0.0;
+                        oldc[i] = 0u;
+                        if (ok) {
+                            if (wtile) oldw[i] = part[row * ncol + c];
+                            else oldc[i] = cpart[row * nclass + c];
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        const uint32_t pl = 4 * (8 * h + i) + lg;
+                        if (pw0 + pl >= num_people) continue;
+                        const int vb = sbuf[pl * 16 + lane_c];
+                        const double v = BF ? (double)__builtin_bit_cast(float, vb) : (double)vb;
+                        const uint64_t row = prow + pl;
+                        if (wtile) part[row * ncol + c] = oldw[i] + v * sc;
+                        else cpart[row * nclass + c] = oldc[i] + (uint32_t)v;
                     }
                 }
             }
@@ -509,7 +532,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(BF ?
     // [c0, c0 + 4, c1, c1 + 4] and one v_perm_b32 maps them to two BF16
     // dosages (low bytes 00 00 80 00 and high bytes 00 00 3F 40 for codes
     // 00 01 10 11: 0, 0, 1.0, 2.0), another to the missing indicator.
-    auto compute_bf = [&](Acc &acc, auto nwc, uint32_t par, int buf, const u32x4_t &X,
+    auto compute_bf = [&](Acc &acc, auto nwc, auto par, int buf, const u32x4_t &X,
                           const u32x4_t &F) __attribute__((always_inline)) {
         if constexpr (BF) {
         constexpr int NW = decltype(nwc)::value;
@@ -526,7 +549,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(BF ?
         for (int h = 0; h < 2; ++h) {
             u32x4_t bv[TMAX];
 #pragma unroll
-            for (int j = 0; j < TMAX; ++j) bv[j] = btile[par][buf][((set * TMAX + j) * 2 + h) * 64 + lane];
+            for (int j = 0; j < TMAX; ++j) bv[j] = btile(par)[buf][((set * TMAX + j) * 2 + h) * 64 + lane];
 #pragma unroll
             for (int tt = 0; tt < 4; ++tt) {
                 BfFrag A, M;
@@ -576,7 +599,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(BF ?
     // one stage's decode with the previous stage's MFMAs; tile slots past
     // ntile are computed on a duplicated fragment and never flushed.  With
     // nwc < 0 the layout is read at run time (per-tile branches).
-    auto compute = [&](Acc &acc, auto nwc, uint32_t par, int buf) __attribute__((always_inline)) {
+    auto compute = [&](Acc &acc, auto nwc, auto par, int buf) __attribute__((always_inline)) {
         constexpr int NW = decltype(nwc)::value;
         const uint32_t *t = tile[buf];
         const u32x4_t X = *(const u32x4_t *)&t[tile_dword(pwave * 16 + li, 4 * lg)];
@@ -587,7 +610,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(BF ?
         } else {
         u32x4_t bv[TMAX];
 #pragma unroll
-        for (int j = 0; j < TMAX; ++j) bv[j] = btile[par][buf][(set * TMAX + j) * 64 + lane];
+        for (int j = 0; j < TMAX; ++j) bv[j] = btile(par)[buf][(set * TMAX + j) * 64 + lane];
         // person tile by person tile: only one tile's selectors and operands
         // are live at a time
 #pragma unroll
@@ -641,14 +664,14 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(BF ?
     // row loads and B-fragment DMA are issued before the previous batch is
     // computed, so their latency overlaps the MFMAs; a batch is stored to
     // LDS only once its loads have landed.
-    auto issue_batch = [&](uint32_t par, uint32_t i) __attribute__((always_inline)) {
+    auto issue_batch = [&](uint32_t i) __attribute__((always_inline)) {
         // stages past the split are loaded clamped and never stored
         if (!loader) return;
         [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
-            (issue(par, I, i + I), ...);
+            (issue(I, i + I), ...);
         }(std::make_integer_sequence<int, D>{});
     };
-    auto fetch_batch = [&](uint32_t par, uint32_t i) __attribute__((always_inline)) {
+    auto fetch_batch = [&](auto par, uint32_t i) __attribute__((always_inline)) {
         [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
             (fetch_b(par, I, i + I < last_i ? i + I : last_i), ...);
         }(std::make_integer_sequence<int, D>{});
@@ -656,36 +679,47 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(BF ?
     auto main_loop = [&](auto nwc) __attribute__((always_inline)) {
         Acc acc;   // one accumulator set per copy of the loop
         zero_acc(acc);
-        uint32_t par = 0;
+        using P0 = std::integral_constant<int, 0>;
+        using P1 = std::integral_constant<int, 1>;
         uint32_t i0 = 0;
-        issue_batch(par, i0);
-        fetch_batch(par, i0);
-        // Outer loop over scale blocks, inner loop over batches of the block:
-        // the flush sits after the inner loop on every path, so the
-        // accumulators carried by the inner loop are only ever MFMA results and
-        // stay in accumulation registers.
+        issue_batch(i0);
+        fetch_batch(P0{}, i0);
+        // One batch whose B fragments sit in btile(par): store its rows,
+        // then (after the barrier that also retires the previous batch's
+        // reads of the other buffer) issue the next batch's row loads and
+        // fragment DMA before computing this one, so both have a whole
+        // batch of MFMAs to land.  Batches past the split's last stage
+        // (the second of a pair at the split's end) store and compute
+        // nothing.
+        auto batch = [&](auto par) __attribute__((always_inline)) {
+            constexpr int NP = 1 - decltype(par)::value;
+            if (loader) {
+                [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
+                    ((i0 + I < nst ? (store(I, I), 0) : 0), ...);
+                }(std::make_integer_sequence<int, D>{});
+            }
+            dma_lds_barrier();   // this batch's B fragments landed and its tiles are stored
+            issue_batch(i0 + D);
+            fetch_batch(std::integral_constant<int, NP>{}, i0 + D);
+            [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
+                ((i0 + I < nst ? (compute(acc, nwc, par, I), 0) : 0), ...);
+            }(std::make_integer_sequence<int, D>{});
+            lds_barrier_i8();   // every wave is done with the tiles before the next batch stores
+            i0 += D;
+        };
+        // Outer loop over scale blocks, inner loop over pairs of batches of
+        // the block (even batch indices use btile0, odd ones btile1; a block
+        // starts at an even index): the flush sits after the inner loop on
+        // every path, so the accumulators carried by the inner loop are only
+        // ever MFMA results and stay in accumulation registers.
         bool first = true;
         do {   // nst > 0
             const uint64_t blk = (st0 + (uint64_t)i0 * kI8Stage) / kFlush;
-            const uint64_t blk_end = ((blk + 1) * kFlush - st0) / kI8Stage;   // a multiple of D
+            const uint64_t blk_end = ((blk + 1) * kFlush - st0) / kI8Stage;   // a multiple of 2 D
             const uint32_t i_end = blk_end < nst ? (uint32_t)blk_end : nst;
             do {   // i0 < i_end on entry: the block holds stage i0
-                if (loader) {
-                    [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
-                        ((i0 + I < nst ? (store(I, I), 0) : 0), ...);
-                    }(std::make_integer_sequence<int, D>{});
-                }
-                dma_lds_barrier();   // this batch's B fragments landed and its tiles are stored
-                issue_batch(par ^ 1u, i0 + D);
-                [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
-                    ((i0 + I < nst ? (compute(acc, nwc, par, I), 0) : 0), ...);
-                }(std::make_integer_sequence<int, D>{});
-                // after this wave's last read of the batch's B fragments: the
-                // DMA into the other half never overlaps an LDS read in flight
-                fetch_batch(par ^ 1u, i0 + D);
-                lds_barrier_i8();   // every wave is done with the tiles before the next batch stores
-                par ^= 1u;
-                i0 += D;
+                batch(P0{});
+                batch(P1{});
             } while (i0 < i_end);
             flush(acc, blk, first, false);
             first = false;
