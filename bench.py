@@ -166,9 +166,51 @@ def measure_unpack(d_bed, N, V, bpv, stride, reps, workload):
     log(f"unpack: {vu} variants, {ms:.3f} ms, {gbs:.1f} GB/s algorithmic")
     del out
     traffic, src = profiled_traffic("k_unpack", workload, "", alg, ms)
+    ceil = hbm_ceilings(d_bed, min(vu * stride, 16 << 30), stream, reps)
     return {"kernel": "k_unpack", "variants": vu, "people": N, "ms": round(ms, 4), "algorithmic_bytes": alg,
             "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "measured_ceilings_gbs": ceil,
+            "frac_of_expand_ceiling": round(gbs / ceil["expand_1to4"], 4) if ceil else None,
             "traffic": traffic, "traffic_source": src}
+
+
+def hbm_ceilings(d_src, nbytes, stream, reps):
+    """Rates of the HBM probe kernels (gnm_hbm_probe) on nbytes of resident
+    data: plain 16-byte reads, the unpack stage's non-temporal 16-byte
+    stores, and a 1:4 expanding copy (every source dword to 16 output
+    bytes, the unpack stage's ratio), in GB/s of bytes moved (read +
+    written), with events on the launch stream."""
+    import ctypes
+
+    import torch
+
+    from gnomon_amd import _lib
+
+    L = _lib.lib()
+    nbytes = nbytes // (1 << 20) * (1 << 20)
+    free, _ = torch.cuda.mem_get_info()
+    nbytes = int(min(nbytes, (free * 0.8) // 4 // (1 << 20) * (1 << 20)))
+    if nbytes < (64 << 20):
+        return None
+    dst = torch.empty(4 * nbytes, dtype=torch.uint8, device="cuda")
+    out = {}
+    for kind, name, moved in ((0, "read", nbytes), (1, "write", nbytes), (2, "expand_1to4", 5 * nbytes)):
+        def launch():
+            _lib.check(L.gnm_hbm_probe(kind, ctypes.c_void_p(d_src.data_ptr()), nbytes,
+                                       ctypes.c_void_p(dst.data_ptr()), dst.numel(),
+                                       ctypes.c_void_p(stream.cuda_stream)))
+        launch()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            launch()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = round(moved / (ms / 1e3) / 1e9, 1)
+    del dst
+    log(f"HBM probes on {nbytes / 2**30:.1f} GiB: {out} GB/s")
+    return out
 
 
 def _errors(S, S64):
@@ -557,6 +599,8 @@ def main():
     ap.add_argument("--launch", default="", metavar="TP,BS",
                     help="launch shape of the headline path's MFMA kernel (tiles per pass, batch stages) instead "
                          "of the default")
+    ap.add_argument("--unpack-only", action="store_true",
+                    help="only measure the standalone unpack stage and the HBM probe kernels on the workload's rows")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise copy of MIB MiB (read + write) three times; used to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters")
@@ -599,6 +643,11 @@ def main():
     synth.synth_rows_device(w, d_bed.data_ptr(), stride, 0, V)
     torch.cuda.synchronize()
     log(f"device .bed generated: {V * bpv / 1e9:.2f} GB ({time.time() - t0:.1f}s)")
+
+    if args.unpack_only:
+        unpack = measure_unpack(d_bed, N, V, bpv, stride, max(3, min(args.steps, 10)), w.name)
+        print(json.dumps({"workload": w.name, "unpack_stage": unpack}), flush=True)
+        return
 
     def run_path(precision):
         t1 = time.time()

@@ -1106,6 +1106,23 @@ gnm_status gnm_unpack_device(const uint8_t *d_rows, uint64_t bytes_per_variant, 
     }
 }
 
+gnm_status gnm_hbm_probe(int kind, const void *d_src, uint64_t src_bytes, void *d_dst, uint64_t dst_bytes,
+                         void *stream) {
+    try {
+        const uintptr_t a = reinterpret_cast<uintptr_t>(d_src), b = reinterpret_cast<uintptr_t>(d_dst);
+        if (kind < 0 || kind > 2) return fail(GNM_ERR_INVALID_ARG, "probe kind must be 0 (read), 1 (write) or 2 (expand)");
+        if (src_bytes == 0 || src_bytes % 16) return fail(GNM_ERR_INVALID_ARG, "probe size must be a positive multiple of 16");
+        if (!d_dst || (b & 15u)) return fail(GNM_ERR_INVALID_ARG, "probe output must be a 16-byte aligned device pointer");
+        if (kind != 1 && (!d_src || (a & 15u))) return fail(GNM_ERR_INVALID_ARG, "probe input must be a 16-byte aligned device pointer");
+        const uint64_t need = kind == 0 ? (uint64_t)256 * 16 * 256 * 4 : (kind == 1 ? src_bytes : 4 * src_bytes);
+        if (dst_bytes < need) return fail(GNM_ERR_INVALID_ARG, "probe output buffer too small");
+        GNM_HIP(launch_hbm_probe(kind, (const uint8_t *)d_src, src_bytes, (uint8_t *)d_dst, (hipStream_t)stream));
+        return GNM_OK;
+    } catch (...) {
+        return fail(GNM_ERR_INTERNAL, "exception in gnm_hbm_probe");
+    }
+}
+
 gnm_status gnm_set_launch_config(gnm_ctx *ctx, uint32_t tiles_per_pass, uint32_t batch_stages) {
     if (!ctx) return fail(GNM_ERR_INVALID_ARG, "ctx is NULL");
     if (!is_int8(ctx->precision)) {

@@ -724,4 +724,77 @@ hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t value, hipStream_t 
     return hipGetLastError();
 }
 
+// HBM ceiling probes: the access shapes of the unpack stage without its
+// decode, so that its rate can be read against what the memory system
+// delivers for the same shapes.  Grid-stride over 16-byte elements (n16 =
+// bytes / 16 of the source) or dwords (n4).
+//   read:   16-byte buffer loads, folded into one dword per thread
+//   write:  16-byte non-temporal buffer stores (the unpack stage's stores)
+//   expand: every source dword becomes 16 bytes of output (1:4, the
+//           unpack stage's ratio of packed bytes to tile bytes)
+// Each thread keeps four independent accesses in flight per iteration.
+__global__ __launch_bounds__(256) void k_probe_read(const uint8_t *__restrict__ src, uint64_t n16,
+                                                     uint32_t *__restrict__ sink) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (uint64_t)gridDim.x * blockDim.x;
+    u32x4_t acc = {0u, 0u, 0u, 0u};
+    for (uint64_t i = t; i < n16; i += 4 * nt) {
+        u32x4_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t j = i + u * nt;
+            v[u] = j < n16 ? *(const u32x4_t *)(src + 16 * j) : u32x4_t{0u, 0u, 0u, 0u};
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc ^= v[u];
+    }
+    sink[t] = acc[0] ^ acc[1] ^ acc[2] ^ acc[3];
+}
+__global__ __launch_bounds__(256) void k_probe_write(uint8_t *__restrict__ dst, uint64_t n16, uint32_t value) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (uint64_t)gridDim.x * blockDim.x;
+    const u32x4_t v = {value, value ^ 1u, value ^ 2u, value ^ 3u};
+    for (uint64_t i = t; i < n16; i += 4 * nt) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t j = i + u * nt;
+            if (j >= n16) break;
+            const __amdgpu_buffer_rsrc_t r =
+                __builtin_amdgcn_make_buffer_rsrc(dst + 16 * (j - j % 65536), 0, 0x7FFFFFFF, kRsrcWord3);
+            __builtin_amdgcn_raw_buffer_store_b128(v, r, (uint32_t)(16 * (j % 65536)), 0, kStoreNT);
+        }
+    }
+}
+__global__ __launch_bounds__(256) void k_probe_expand(const uint8_t *__restrict__ src, uint64_t n4,
+                                                       uint8_t *__restrict__ dst) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = t; i < n4; i += 4 * nt) {
+        uint32_t x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t j = i + u * nt;
+            x[u] = j < n4 ? *(const uint32_t *)(src + 4 * j) : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t j = i + u * nt;
+            if (j >= n4) break;
+            const u32x4_t o = {x[u] & 0x03030303u, (x[u] >> 2) & 0x03030303u, (x[u] >> 4) & 0x03030303u,
+                               (x[u] >> 6) & 0x03030303u};
+            const __amdgpu_buffer_rsrc_t r =
+                __builtin_amdgcn_make_buffer_rsrc(dst + 16 * (j - j % 65536), 0, 0x7FFFFFFF, kRsrcWord3);
+            __builtin_amdgcn_raw_buffer_store_b128(o, r, (uint32_t)(16 * (j % 65536)), 0, kStoreNT);
+        }
+    }
+}
+
+hipError_t launch_hbm_probe(int kind, const uint8_t *src, uint64_t src_bytes, uint8_t *dst, hipStream_t stream) {
+    const dim3 grid(256 * 16), block(256);   // 16 workgroups per CU, grid-stride
+    if (kind == 0)
+        hipLaunchKernelGGL(k_probe_read, grid, block, 0, stream, src, src_bytes / 16, (uint32_t *)dst);
+    else if (kind == 1)
+        hipLaunchKernelGGL(k_probe_write, grid, block, 0, stream, dst, src_bytes / 16, 0x5A5A5A5Au);
+    else
+        hipLaunchKernelGGL(k_probe_expand, grid, block, 0, stream, src, src_bytes / 4, dst);
+    return hipGetLastError();
+}
+
 }  // namespace gnm

@@ -207,6 +207,16 @@ gnm_status gnm_unpack_device(const uint8_t *d_rows, uint64_t bytes_per_variant,
                              uint64_t num_variants, uint64_t num_people, const uint8_t *d_flip,
                              uint8_t *d_dosage, uint32_t *d_missing_per_person, void *stream);
 
+/* HBM ceiling probes for the unpack stage's roofline (no reference
+ * counterpart: measurement only).  kind 0 reads src_bytes from d_src (output:
+ * one dword per probe thread, d_dst of at least 4 MiB); kind 1 writes
+ * src_bytes to d_dst with the unpack stage's non-temporal 16-byte stores
+ * (d_src unused); kind 2 expands every source dword into 16 output bytes
+ * (reads src_bytes, writes 4 * src_bytes), the unpack stage's 1:4 ratio.
+ * src_bytes: a positive multiple of 16; pointers 16-byte aligned. */
+gnm_status gnm_hbm_probe(int kind, const void *d_src, uint64_t src_bytes, void *d_dst, uint64_t dst_bytes,
+                         void *stream);
+
 /* Dense effect-allele forward, S[p,k] = sum_v g_pv W[v,k] with
  * g = d (flip 0) or 2-d (flip 1), missing -> 0; missing_per_person[p] =
  * number of missing calls.  W is [num_variants][num_scores] f32, S is

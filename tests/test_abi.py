@@ -86,6 +86,20 @@ def test_invalid_arguments_rejected_without_device():
     assert not ctx.value
 
 
+def test_hbm_probe_rejects_bad_arguments_without_device():
+    L = _lib.lib()
+    buf = ctypes.c_void_p(1 << 20)   # never dereferenced: every call below fails its checks first
+    assert L.gnm_hbm_probe(3, buf, 4096, buf, 1 << 30, None) == _lib.GNM_ERR_INVALID_ARG
+    assert "kind" in _lib.last_error()
+    assert L.gnm_hbm_probe(0, buf, 4100, buf, 1 << 30, None) == _lib.GNM_ERR_INVALID_ARG
+    assert L.gnm_hbm_probe(0, buf, 0, buf, 1 << 30, None) == _lib.GNM_ERR_INVALID_ARG
+    assert L.gnm_hbm_probe(2, buf, 4096, ctypes.c_void_p((1 << 20) + 4), 1 << 30, None) == _lib.GNM_ERR_INVALID_ARG
+    assert L.gnm_hbm_probe(2, buf, 4096, buf, 4 * 4096 - 16, None) == _lib.GNM_ERR_INVALID_ARG
+    assert "too small" in _lib.last_error()
+    assert L.gnm_hbm_probe(0, buf, 4096, buf, 1 << 20, None) == _lib.GNM_ERR_INVALID_ARG   # sink < 4 MiB
+    assert L.gnm_hbm_probe(0, None, 4096, buf, 1 << 30, None) == _lib.GNM_ERR_INVALID_ARG
+
+
 def test_small_problem_declines_and_no_device_is_reported(monkeypatch):
     L = _lib.lib()
     csr = csr_from_dense(np.ones((4, 2), np.float32), np.zeros(4, bool))
