@@ -143,13 +143,13 @@ __device__ __forceinline__ uint32_t bf_miss_pair(uint32_t x) {
 }  // namespace
 
 template <int TMAX, int WAVES, int D, bool VEC, bool IDENT, int SETS, bool BF>
-// The int8 3-tile, 2-stage instance is asked to fit four waves per SIMD
-// (128 registers: a few values move to accumulation registers, none to
-// memory); the BF16 instances two waves per SIMD (256 registers).
+// The 3-tile, 2-stage instances (int8 and BF16) are asked to fit four waves
+// per SIMD (128 registers; a few values outside the MFMA blocks are spilled),
+// the other BF16 instances two waves per SIMD (256 registers).
 // SETS > 1: the waves form SETS sets over the same people, each set with its
 // own group of TMAX tiles, so one row read serves SETS groups (the first
 // set loads and stores the rows for all).
-__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(BF ? 2 : (TMAX == 3 && D == 2 ? 4 : 1)))) void k_score_mx(DevicePlan plan, const uint8_t *__restrict__ rows,
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX == 3 && D == 2 ? 4 : (BF ? 2 : 1)))) void k_score_mx(DevicePlan plan, const uint8_t *__restrict__ rows,
                                                          uint64_t stride, uint64_t bpv, uint64_t v0, uint64_t p0,
                                                          uint64_t p1, uint64_t split_base, uint64_t rps,
                                                          uint32_t splits, uint32_t person_tiles,
@@ -435,35 +435,26 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(BF ?
             const uint32_t nc = wtile ? ncol : nclass;
             if ((uint32_t)j < ntile && c < nc) {
                 const double sc = (wtile && !empty) ? plan.scales[blk * ncol + c] : 0.0;
-                // 16 people per lane in two halves of 8: a half's earlier
-                // partials are all loaded before any is updated, so a
-                // read-modify-write flush waits for one load latency per
-                // half instead of one per person
+                // the first flush of a workgroup stores its partials; later
+                // ones add to them with no-return device atomics (each
+                // partial has one writer, so the sum is the same FP64 add a
+                // read-modify-write would do, with no load latency and no
+                // registers held for it)
 #pragma unroll 1
-                for (int h = 0; h < 2; ++h) {
-                    double oldw[8];
-                    uint32_t oldc[8];
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) {
-                        const uint32_t pl = 4 * (8 * h + i) + lg;   // person of the wave, 0..63
-                        const bool ok = pw0 + pl < num_people && !first;
-                        const uint64_t row = prow + pl;
-                        oldw[i] = 0.0;
-                        oldc[i] = 0u;
-                        if (ok) {
-                            if (wtile) oldw[i] = part[row * ncol + c];
-                            else oldc[i] = cpart[row * nclass + c];
-                        }
-                    }
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) {
-                        const uint32_t pl = 4 * (8 * h + i) + lg;
-                        if (pw0 + pl >= num_people) continue;
-                        const int vb = sbuf[pl * 16 + lane_c];
-                        const double v = BF ? (double)__builtin_bit_cast(float, vb) : (double)vb;
-                        const uint64_t row = prow + pl;
-                        if (wtile) part[row * ncol + c] = oldw[i] + v * sc;
-                        else cpart[row * nclass + c] = oldc[i] + (uint32_t)v;
+                for (int i = 0; i < 16; ++i) {
+                    const uint32_t pl = 4 * i + lg;  // person of the wave, 0..63
+                    if (pw0 + pl >= num_people) continue;
+                    const int vb = sbuf[pl * 16 + lane_c];
+                    const double v = BF ? (double)__builtin_bit_cast(float, vb) : (double)vb;
+                    const uint64_t row = prow + pl;
+                    if (wtile) {
+                        double *dd = part + row * ncol + c;
+                        if (first) *dd = v * sc;
+                        else unsafeAtomicAdd(dd, v * sc);
+                    } else {
+                        uint32_t *dd = cpart + row * nclass + c;
+                        if (first) *dd = (uint32_t)v;
+                        else atomicAdd(dd, (uint32_t)v);
                     }
                 }
             }
@@ -487,7 +478,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(BF ?
             if (pw0 + pl < num_people) {
                 const uint32_t x = (uint32_t)(sbuf[pl] + sbuf[64 + pl] + sbuf[128 + pl] + sbuf[192 + pl]);
                 uint32_t *dd = cpart + (prow + pl) * nclass;
-                *dd = first ? x : *dd + x;
+                if (first) *dd = x;
+                else atomicAdd(dd, x);
             }
         }
         // staging inside the .bed tiles: no wave may store the next batch's
