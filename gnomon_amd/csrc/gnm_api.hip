@@ -404,7 +404,8 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
         // themselves: one membership class)
         std::vector<uint8_t> flipbyte(Rpad, 0);
         const uint8_t sel = int8 ? kFlipSelI8 : kFlipSelBf;
-        const bool vc = c->precision != GNM_PRECISION_FP32 && c->nclass == 1;
+        const bool vc = c->precision != GNM_PRECISION_FP32 && c->nclass == 1 &&
+                        vcount_pays(int8, (uint32_t)((c->limbs * K + 15) / 16));
         for (uint64_t r = 0; r < R; ++r) {
             if ((flipmask[r >> 5] >> (r & 31)) & 1u) flipbyte[r] = sel;
             if (vc && prog_count_row[r]) flipbyte[r] |= 0x80u;
@@ -421,8 +422,10 @@ gnm_status build_context(const gnm_problem *p, gnm_ctx *c) {
     pl.ncol = mfma_weights ? (uint32_t)(L * K) : 0;
     pl.wtiles = (pl.ncol + 15) / 16;
     // one membership class: the MFMA kernels count missing calls on the
-    // vector ALUs (no count tile)
-    pl.vcount = (mfma_weights && c->nclass == 1) ? 1u : 0u;
+    // vector ALUs (no count tile) where the cost model says a count tile
+    // costs more (it does when it would need a tile group of its own, not
+    // when it fills a slot the weight tiles leave free)
+    pl.vcount = (mfma_weights && c->nclass == 1 && vcount_pays(int8, pl.wtiles)) ? 1u : 0u;
     pl.ctiles = (mfma_counts && !pl.vcount) ? (c->nclass + 15) / 16 : 0;
     const uint32_t tiles = pl.wtiles + pl.ctiles;
 
@@ -1351,8 +1354,10 @@ gnm_status gnm_dense_forward_device(const uint8_t *d_rows, uint64_t bytes_per_va
         } else {
             pl.ncol = (uint32_t)(L * K);
             pl.wtiles = (pl.ncol + 15) / 16;
-            pl.ctiles = 0;
-            pl.vcount = 1;   // one class, every row a count row
+            // one class, every row a count row: counted on the vector ALUs or
+            // in one count tile, whichever the cost model prefers
+            pl.vcount = vcount_pays(int8, pl.wtiles) ? 1u : 0u;
+            pl.ctiles = pl.vcount ? 0u : 1u;
             const uint32_t tiles = pl.wtiles + pl.ctiles;
             const uint64_t nst = Rpad / kI8KStep;   // 64-row stages
             const uint64_t fr = (uint64_t)flush_rows(int8);

@@ -24,6 +24,10 @@ static uint32_t bf16_default_tiles_per_pass(uint32_t ntiles) {
     return tmax;
 }
 
+uint64_t bf16_default_cost(uint32_t ntiles) {
+    return bf16_pass_cost(ntiles, bf16_default_tiles_per_pass(ntiles));
+}
+
 // 3-, 8- and 9-tile groups run 8 waves (512 people, whole 128-byte lines of
 // each row; the B fragments of 8- and 9-tile groups, 16-18 KB per stage
 // double-buffered, leave room for one workgroup per CU, so 8 waves keep two

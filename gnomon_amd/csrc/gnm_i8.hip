@@ -23,6 +23,10 @@ uint32_t i8_default_tiles_per_pass(uint32_t ntiles) {
     return tmax;
 }
 
+uint64_t i8_default_cost(uint32_t ntiles) {
+    return i8_pass_cost(ntiles, i8_default_tiles_per_pass(ntiles));
+}
+
 // Variant splits of the int8 kernel: enough that all tile groups together
 // launch about 12k workgroups (~48 per CU, so that three stay resident per
 // CU through the tail and many row strips are read at once), a multiple of

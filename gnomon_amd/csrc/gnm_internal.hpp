@@ -125,6 +125,18 @@ hipError_t launch_score_i8(const DevicePlan &plan, const uint8_t *rows, uint64_t
                            uint32_t tiles_per_pass = 0, uint32_t batch_stages = 0);
 // variant splits of the int8 kernel for num_people and ntiles tiles
 uint32_t i8_splits(uint64_t num_people, uint32_t ntiles);
+// cost-model units per wave and stage of a scoring pass over ntiles tiles at
+// the default tiles per pass
+uint64_t i8_default_cost(uint32_t ntiles);
+uint64_t bf16_default_cost(uint32_t ntiles);
+// One membership class: count its missing calls on the vector ALUs (about 30
+// instructions per wave and stage in one tile group) rather than in a count
+// tile, when the cost model says the extra tile costs more.
+inline bool vcount_pays(bool int8, uint32_t wtiles) {
+    const uint64_t valu = int8 ? i8_default_cost(wtiles) : bf16_default_cost(wtiles);
+    const uint64_t ctile = int8 ? i8_default_cost(wtiles + 1) : bf16_default_cost(wtiles + 1);
+    return valu + 120 < ctile;
+}
 // Split-BF16 scoring (gnm_bf16.hip): the same kernel with BF16 limbs (B
 // fragments [64-row stage][tile][k-step h][lane][8 bf16], rows 16 g + 8 h + j
 // of the stage for lane (n, g)), the same row and stride conditions, one
