@@ -573,7 +573,15 @@ gnm_status score_device_rows(gnm_ctx *c, const uint8_t *d_rows, uint64_t stride,
         bpv = (c->N + 3) / 4;
     }
     const uint64_t p0 = c->var_prog_off[v0], p1 = c->var_prog_off[v0 + n];
-    const uint32_t S = c->splits;
+    // variant splits of this launch: the context's count, but on the MFMA
+    // paths no more than one per kMinSplitRows program rows of the block (at
+    // least 8, a multiple of 8), so that a block of few rows does not launch
+    // thousands of workgroups of a stage or two each
+    uint32_t S = c->splits;
+    if (c->precision != GNM_PRECISION_FP32) {
+        const uint64_t want = ((p1 - p0 + kMinSplitRows - 1) / kMinSplitRows + 7) / 8 * 8;
+        S = (uint32_t)std::min<uint64_t>(S, std::max<uint64_t>(want, 8));
+    }
     if (p1 == p0) {   // no variant of the block is in any score: nothing to add
         GNM_HIP(hipEventRecord(c->ev_main0, c->stream));
         GNM_HIP(hipEventRecord(c->ev_main1, c->stream));

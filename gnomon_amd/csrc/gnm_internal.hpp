@@ -23,6 +23,8 @@ constexpr int kI8FlushRows = 65536;
 // over 32768 rows (32768 * 510 = 16711680).  (The split-BF16 adjoint keeps
 // kFlushRows-person blocks.)
 constexpr int kBfFlushRows = 32768;
+// fewest program rows per variant split of a scoring launch on the MFMA paths
+constexpr uint64_t kMinSplitRows = 4096;
 constexpr int flush_rows(bool int8) { return int8 ? kI8FlushRows : kBfFlushRows; }
 constexpr int kLimbBits = 8;
 constexpr int kFp32MaxClasses = 4;     // membership classes counted in-kernel by the FP32 kernel

@@ -261,6 +261,10 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
     const uint32_t fast_lo = (uint32_t)((r_begin - st0 + kI8Stage - 1) / kI8Stage);
     const uint32_t fast_hi = (IDENT && span_full && nonempty) ? (uint32_t)((r_end - st0) / kI8Stage) : 0u;
     const uint32_t last_i = nonempty ? (uint32_t)((last_st - st0) / kI8Stage) : 0u;
+    // the split's row range relative to st0, and its first row's variant
+    // (non-identity rows)
+    const uint32_t rb_rel = (uint32_t)(r_begin - st0), re_rel = nonempty ? (uint32_t)(r_end - st0) : rb_rel;
+    const uint32_t var_begin = (!IDENT && nonempty) ? plan.prog_var[r_begin] : (uint32_t)v0;
     // B fragments: wave w fetches fragments u = w, w + WAVES, ... of the
     // sets' kBU fragments (tile u / KS, k-step u % KS) by LDS DMA (lane l's
     // 16 bytes land at btile(par)[d][64 u + l]); tiles past tile_end are fetched
@@ -324,14 +328,29 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
         const uint64_t st = st_req < last_st ? st_req : last_st;
         const bool same = st == st_req;
         uint32_t keep = 0;
+        if constexpr (IDENT) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint64_t r = st + rg4 + k;
-            const bool ok = same && r >= r_begin && r < r_end && lcol < bpv;
-            keep |= ok ? (1u << k) : 0u;
-            const uint64_t rc = r < r_begin ? r_begin : (r >= r_end ? r_end - 1 : r);
-            const uint64_t var = IDENT ? (rc - p0) : (uint64_t)(plan.prog_var[rc] - (uint32_t)v0);
-            ra[d][k] = load_row(var, lcol);
+            for (int k = 0; k < 4; ++k) {
+                const uint64_t r = st + rg4 + k;
+                const bool ok = same && r >= r_begin && r < r_end && lcol < bpv;
+                keep |= ok ? (1u << k) : 0u;
+                const uint64_t rc = r < r_begin ? r_begin : (r >= r_end ? r_end - 1 : r);
+                ra[d][k] = load_row(rc - p0, lcol);
+            }
+        } else {
+            // the variants of the thread's four rows in one 16-byte load
+            // (st + rg4 + 3 < Rpad: st is at most the split's last stage);
+            // rows outside the split read the split's first variant instead,
+            // a row of this block, and are masked at store time.  Row tests
+            // in 32 bits, relative to the split's first stage.
+            const u32x4_t pv = *(const u32x4_t *)(plan.prog_var + st + rg4);
+            const uint32_t rel = (uint32_t)(st - st0) + rg4;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const bool ok = same && rel + k >= rb_rel && rel + k < re_rel && lcol < bpv;
+                keep |= ok ? (1u << k) : 0u;
+                ra[d][k] = load_row((uint64_t)((ok ? pv[k] : var_begin) - (uint32_t)v0), lcol);
+            }
         }
         rk[d] = keep;
         rfull[d] = false;
