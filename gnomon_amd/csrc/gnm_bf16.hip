@@ -28,11 +28,14 @@ uint64_t bf16_default_cost(uint32_t ntiles) {
     return bf16_pass_cost(ntiles, bf16_default_tiles_per_pass(ntiles));
 }
 
-// 3-, 8- and 9-tile groups run 8 waves (512 people, whole 128-byte lines of
-// each row; the B fragments of 8- and 9-tile groups, 16-18 KB per stage
+// 3- and 9-tile groups run 8 waves (512 people, whole 128-byte lines of
+// each row; the 9-tile group's 18 KB of B fragments per stage,
 // double-buffered, leave room for one workgroup per CU, so 8 waves keep two
-// per SIMD); 5- and 6-tile groups run 4 waves, two workgroups per CU
-static uint32_t bf16_waves(uint32_t tmax) { return (tmax == 3 || tmax == 8 || tmax == 9) ? 8u : 4u; }
+// per SIMD); 5-, 6- and 8-tile groups run 4 waves, two workgroups per CU
+// (the 8-tile group's flush staged in its idle B buffer: 74 KB of LDS).  At
+// 100k x 1M x 64 in bf16x4 the 4-wave 8-tile group took 39.4 ms against
+// 41.0 ms with 8 waves.
+static uint32_t bf16_waves(uint32_t tmax) { return (tmax == 3 || tmax == 9) ? 8u : 4u; }
 
 // Variant splits: about 12k workgroups over all tile groups, a multiple of 8
 // (whole splits per XCD), 8..32 (the FP64 partials grow with the count).
@@ -83,7 +86,7 @@ hipError_t launch_score_bf16(const DevicePlan &plan, const uint8_t *rows, uint64
     if (tp == 3) GNM_LAUNCH_BF_VI(3, 8);
     else if (tp == 5) GNM_LAUNCH_BF_VI(5, 4);
     else if (tp == 6) GNM_LAUNCH_BF_VI(6, 4);
-    else if (tp == 8) GNM_LAUNCH_BF_VI(8, 8);
+    else if (tp == 8) GNM_LAUNCH_BF_VI(8, 4);
     else GNM_LAUNCH_BF_VI(9, 8);
 #undef GNM_LAUNCH_BF_VI
 #undef GNM_LAUNCH_BF

@@ -411,8 +411,13 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
     // as counts.
     // the staging buffer shares the .bed tiles' LDS when they are large
     // enough (no tile is in use while a block is flushed)
+    // Otherwise it is btile1 when that is large enough: a flush follows an
+    // odd batch (or no batch), so btile1 has no DMA in flight, its last reads
+    // retired at that batch's closing barrier, and the next DMA into it (the
+    // next block's first batch) comes after the flush's closing barrier.
     constexpr bool kStageInTile = D * kTileWords >= WAVES * 64 * 16;
-    __shared__ int stage_own[kStageInTile ? 1 : WAVES * 64 * 16];
+    constexpr bool kStageInB = !kStageInTile && D * kBU * 64 * 4 >= WAVES * 64 * 16;
+    __shared__ int stage_own[(kStageInTile || kStageInB) ? 1 : WAVES * 64 * 16];
     auto flush = [&](Acc &acc, uint64_t blk, bool first, bool empty) __attribute__((always_inline)) {
         // opaque to the optimiser, so the addresses below are formed here
         // rather than hoisted out of the row loop into live registers
@@ -422,7 +427,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
         ncol = __builtin_amdgcn_readfirstlane(ncol);
         nclass = __builtin_amdgcn_readfirstlane(nclass);
         tb = __builtin_amdgcn_readfirstlane(tb);
-        int *sbuf = (kStageInTile ? (int *)&tile[0][0] : stage_own) + wave * (64 * 16);
+        int *sbuf = (kStageInTile ? (int *)&tile[0][0] : (kStageInB ? (int *)&btile1[0][0] : stage_own)) +
+                    wave * (64 * 16);
         auto flush_tile = [&](auto jc) __attribute__((always_inline)) {
             constexpr int j = decltype(jc)::value;
             __builtin_amdgcn_sched_barrier(0);
@@ -503,7 +509,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
         }
         // staging inside the .bed tiles: no wave may store the next batch's
         // tiles while another still reads its staged sums
-        if constexpr (kStageInTile) lds_barrier_i8();
+        if constexpr (kStageInTile || kStageInB) lds_barrier_i8();
     };
 
     // Missing calls (code 01) of the lane's four people over its 16 rows of
