@@ -101,17 +101,17 @@ def dense_adjoint(R, bed, flip, num_variants, precision="auto"):
 
 
 def _adjoint_precision(precision, bed):
-    """The adjoint's arithmetic for a forward precision: split-int8 for the
-    int8 paths when the rows are 4-byte aligned with a row stride that is a
-    multiple of 4 (what the int8 kernel's dword loads need), else split-BF16
-    (bf16x3 unless bf16x2 is asked for); "auto" is resolved by the library
+    """The adjoint's arithmetic for a forward precision: the same split when
+    the rows are 4-byte aligned with a row stride that is a multiple of 4
+    (what the split-limb adjoint kernel's row loads need); on other rows the
+    int8 paths fall back to bf16x3 and bf16x4 to bf16x3 (the byte-load BF16
+    kernel holds at most 12 limb tiles); "auto" is resolved by the library
     (split-int8 where the rows allow it and its self-test passed)."""
     if precision == "auto":
         return "auto"
-    if precision in ("i8x4", "i8x3"):
-        if bed.data_ptr() % 4 == 0 and bed.shape[1] % 4 == 0:
-            return precision
-        return "bf16x3"
+    aligned = bed.data_ptr() % 4 == 0 and bed.shape[1] % 4 == 0
+    if precision in ("i8x4", "i8x3", "bf16x4"):
+        return precision if aligned else "bf16x3"
     return "bf16x2" if precision == "bf16x2" else "bf16x3"
 
 

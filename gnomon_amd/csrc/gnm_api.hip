@@ -1254,26 +1254,34 @@ gnm_status ws_typed(DenseWorkspace *ws, size_t slot, size_t count, hipStream_t s
     return s;
 }
 
-// Buffers of the split-int8 adjoint, in elements: block exponents (int),
-// R limb fragments (u32), limb scales (f64); the FP64 accumulator is
-// [V][Kp].  R is split per block of kI8FlushRows people.
+// Buffers of the split-int8 / split-BF16 adjoint, in elements: block
+// exponents (int), R limb fragments (u32), limb scales (f64); the FP64
+// accumulator is [V][Kp].  R is split per block of kI8FlushRows (int8) or
+// kBfFlushRows (BF16) people.
 struct AdjointI8Shape {
+    bool bf;
     uint32_t L, used, tiles;
     uint64_t Kp, nks, nblk, exps, frag, scales;
 };
 
 gnm_status adjoint_i8_shape(uint64_t N, uint64_t K, int32_t precision, bool aligned, AdjointI8Shape *sh) {
+    sh->bf = !is_int8(precision);
     sh->L = limbs_of(precision);
     sh->Kp = (K + 15) / 16 * 16;
     sh->used = (uint32_t)(sh->L * (sh->Kp / 16));
     sh->tiles = (sh->used + 3) / 4 * 4;
-    if (sh->tiles > 16) return fail(GNM_ERR_UNSUPPORTED, "the int8 adjoint supports up to 64 scores per call");
+    if (sh->tiles > 16)
+        return fail(GNM_ERR_UNSUPPORTED, sh->bf ? "the split-BF16 adjoint supports up to 16 limb tiles per call "
+                                                  "(64 scores in bf16x4)"
+                                                : "the int8 adjoint supports up to 64 scores per call");
     if (!aligned)
-        return fail(GNM_ERR_UNSUPPORTED, "the int8 adjoint needs 4-byte aligned rows with a stride that is a "
-                                         "multiple of 4 (below 2^25)");
-    const uint64_t nss = ((N + 3) / 4 + 63) / 64;   // super-steps of 256 people
+        return fail(GNM_ERR_UNSUPPORTED, "the split-limb adjoint needs 4-byte aligned rows with a stride that is "
+                                         "a multiple of 4 (below 2^25)");
+    // super-steps of 256 people (int8) or 128 people (BF16), 4 k-steps each
+    const uint64_t nss = ((N + 3) / 4 + (sh->bf ? 31 : 63)) / (sh->bf ? 32 : 64);
+    const uint64_t fr = sh->bf ? (uint64_t)kBfFlushRows : (uint64_t)kI8FlushRows;
     sh->nks = nss * 4;
-    sh->nblk = (N + kI8FlushRows - 1) / kI8FlushRows;
+    sh->nblk = (N + fr - 1) / fr;
     sh->exps = sh->nblk * K;
     sh->frag = sh->nks * sh->tiles * 64 * 4;
     sh->scales = sh->nblk * sh->L * sh->Kp;
@@ -1287,10 +1295,15 @@ gnm_status adjoint_i8_launch(const AdjointI8Shape &sh, const uint8_t *d_rows, ui
                              uint64_t N, const uint8_t *d_flip, const float *d_grad_scores, uint64_t K, int *exps,
                              uint32_t *frag, double *scales, double *acc, float *d_grad_weights, hipStream_t st) {
     GNM_HIP(launch_fill_f64(scales, sh.scales, 0.0, st));
-    GNM_HIP(launch_block_exponent(d_grad_scores, N, K, K, kI8FlushRows, sh.L, true, exps, st));
-    GNM_HIP(launch_adjoint_frag_i8(d_grad_scores, N, K, sh.Kp, sh.L, exps, sh.tiles, sh.used, sh.nks, frag, scales,
-                                   st));
-    GNM_HIP(launch_adjoint_i8(d_rows, stride, V, N, d_flip, frag, scales, sh.tiles, K, sh.Kp, sh.L, acc,
+    GNM_HIP(launch_block_exponent(d_grad_scores, N, K, K, sh.bf ? kBfFlushRows : kI8FlushRows, sh.L, !sh.bf, exps,
+                                  st));
+    if (sh.bf)
+        GNM_HIP(launch_adjoint_frag_bf(d_grad_scores, N, K, sh.Kp, sh.L, exps, sh.tiles, sh.used, sh.nks, frag,
+                                       scales, st));
+    else
+        GNM_HIP(launch_adjoint_frag_i8(d_grad_scores, N, K, sh.Kp, sh.L, exps, sh.tiles, sh.used, sh.nks, frag,
+                                       scales, st));
+    GNM_HIP(launch_adjoint_mx(sh.bf, d_rows, stride, V, N, d_flip, frag, scales, sh.tiles, K, sh.Kp, sh.L, acc,
                               d_grad_weights, st));
     return GNM_OK;
 }
@@ -1423,7 +1436,11 @@ gnm_status gnm_dense_adjoint_device(const uint8_t *d_rows, uint64_t bytes_per_va
             precision = (aligned && resolve_auto(dev) == GNM_PRECISION_I8X4) ? GNM_PRECISION_I8X4
                                                                              : GNM_PRECISION_BF16X3;
         }
-        if (is_int8(precision)) {
+        // split-int8, and split-BF16 on aligned rows: the shared adjoint
+        // kernel; split-BF16 on unaligned rows: the byte-load kernel below
+        const bool bf = precision == GNM_PRECISION_BF16X2 || precision == GNM_PRECISION_BF16X3 ||
+                        precision == GNM_PRECISION_BF16X4;
+        if (is_int8(precision) || (bf && aligned && limbs_of(precision) * ((K + 15) / 16) <= 16)) {
             AdjointI8Shape sh;
             gnm_status s = adjoint_i8_shape(N, K, precision, aligned, &sh);
             if (s != GNM_OK) return s;

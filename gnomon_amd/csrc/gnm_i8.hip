@@ -240,29 +240,93 @@ hipError_t launch_adjoint_frag_i8(const float *R, uint64_t N, uint64_t K, uint64
     return hipGetLastError();
 }
 
-// One workgroup = 4 waves x MT 16-variant tiles.  Per super-step the
-// workgroup's 256 threads load the B fragments of 4 k-steps (16 B x TILES
-// each) and store them to LDS (double-buffered, one barrier per
-// super-step); each lane loads its MT rows' 16 bytes.  Both are loaded one
-// super-step ahead into registers.
-template <int TILES, int MT>
-__global__ __launch_bounds__(256) void k_adjoint_i8(const uint8_t *__restrict__ rows, uint64_t stride, uint64_t V,
+// BF16 B fragments [ks][tile][lane][8 bf16] of R [N][K] for the split-BF16
+// adjoint: column c = limb*Kp + k; k-step ks = 4 S + d covers, for lane
+// group g, the 8 people 128 S + 32 g + 4 (2 d + h) + t (slot 2 t + h, h = 0, 1,
+// t = 0..3), the order in which the adjoint kernel decodes a lane's row
+// bytes 2d and 2d + 1 of its 8-byte share of super-step S.  One thread per
+// (k-step, tile, lane).
+__global__ void k_adjoint_frag_bf(const float *__restrict__ R, uint64_t N, uint64_t K, uint64_t Kp, uint32_t L,
+                                  const int *__restrict__ exps, uint32_t tiles, uint32_t tiles_used, uint64_t nks,
+                                  uint32_t *__restrict__ frag, double *__restrict__ scales) {
+    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= nks * tiles * 64) return;
+    const uint32_t lane = id & 63;
+    const uint32_t tile = (uint32_t)((id >> 6) % tiles);
+    const uint64_t ks = (id >> 6) / tiles;
+    const uint32_t n = lane & 15, g = lane >> 4;
+    const uint64_t c = (uint64_t)tile * 16 + n;
+    const uint32_t limb = (uint32_t)(c / Kp);
+    const uint64_t k = c % Kp;
+    const bool col_ok = tile < tiles_used && k < K;
+    const uint64_t S = ks >> 2, d = ks & 3;
+    uint32_t out[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const uint64_t person = 128 * S + 32 * g + 4 * (2 * d + (j & 1)) + (j >> 1);
+        uint32_t val = 0;
+        if (col_ok && person < N) {
+            const uint64_t blk = person / kBfFlushRows;
+            const float w = R[person * K + k];
+            const int e = exps[blk * K + k];
+            if (w != 0.0f && e != kBadExp) {
+                int32_t dig[4];
+                limb_digits((int64_t)rint(ldexp((double)w, -e)), L, dig);
+                val = bf16_of_int(dig[limb]);
+            }
+        }
+        out[j >> 1] |= val << (16 * (j & 1));
+    }
+    uint32_t *dst = frag + id * 4;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dst[q] = out[q];
+    if (col_ok && g == 0 && (ks * 32) % kBfFlushRows == 0) {
+        const uint64_t blk = ks * 32 / kBfFlushRows;
+        scales[blk * (L * Kp) + c] = limb_scale(exps[blk * K + k], 8 * (int)(L - 1 - limb));
+    }
+}
+
+hipError_t launch_adjoint_frag_bf(const float *R, uint64_t N, uint64_t K, uint64_t Kp, uint32_t L, const int *exps,
+                                  uint32_t tiles, uint32_t tiles_used, uint64_t nks, uint32_t *frag, double *scales,
+                                  hipStream_t stream) {
+    const uint64_t n = nks * tiles * 64;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adjoint_frag_bf, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, R, N, K, Kp, L,
+                       exps, tiles, tiles_used, nks, frag, scales);
+    return hipGetLastError();
+}
+
+// The adjoint dW = G^T R, split-int8 (BF = false) or split-BF16 (BF = true).
+// One workgroup = 4 waves x MT 16-variant tiles.  A super-step covers 256
+// people (int8: each lane loads 16 bytes of its row, four k-steps of
+// v_mfma_i32_16x16x64_i8) or 128 people (BF16: 8 bytes, four k-steps of
+// v_mfma_f32_16x16x32_bf16, each decoding two of the lane's bytes into 8
+// BF16 dosages).  Per super-step the workgroup's 256 threads load the B
+// fragments of its 4 k-steps (16 B x TILES each) and store them to LDS
+// (double-buffered, one barrier per super-step); each lane loads its MT
+// rows' bytes.  Both are loaded one super-step ahead into registers.
+template <int TILES, int MT, bool BF>
+__global__ __launch_bounds__(256) void k_adjoint_mx(const uint8_t *__restrict__ rows, uint64_t stride, uint64_t V,
                                                     uint64_t N, const uint8_t *__restrict__ flip,
                                                     const u32x4_t *__restrict__ frag,
                                                     const double *__restrict__ scales, uint64_t Kp, uint32_t L,
                                                     double *__restrict__ ws) {
     constexpr int kBPer = TILES;                  // B entries (16 B) per thread per super-step
+    constexpr int kSSBytes = BF ? 32 : 64;        // row bytes per super-step
+    constexpr int kLaneBytes = kSSBytes / 4;      // of which each lane group loads
+    typedef std::conditional_t<BF, f32x4_t, i32x4_t> AccV;
+    typedef std::conditional_t<BF, uint64_t, u32x4_t> RowV;
     __shared__ u32x4_t bt[2][4 * TILES * 64];     // [buf][(k-step d, tile j, lane)]
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int li = lane & 15, lg = lane >> 4;
     const uint64_t wv0 = ((uint64_t)blockIdx.x * 4 + wave) * (16 * MT);   // first variant of the wave
     const uint64_t bpv = (N + 3) / 4;
-    const uint64_t nss = (bpv + 63) / 64;                                // super-steps
+    const uint64_t nss = (bpv + kSSBytes - 1) / kSSBytes;                // super-steps
     const uint32_t tpl = (uint32_t)(Kp / 16);                            // tiles per limb
-    // rows of the wave: one descriptor, lane offsets (16mt + i) * stride + 16g;
-    // rows past V read as zero (range check), bytes past the row end belong
-    // to people whose R limbs are zero
+    // rows of the wave: one descriptor, lane offsets (16mt + i) * stride +
+    // kLaneBytes g; rows past V read as zero (range check), bytes past the
+    // row end belong to people whose R limbs are zero
     const uint64_t rem = wv0 < V ? (V - wv0) * stride : 0;
     const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
         (void *)(rows + (wv0 < V ? wv0 : 0) * stride), 0, (int)(rem < 0x7FFFFFFFull ? rem : 0x7FFFFFFFull),
@@ -271,30 +335,39 @@ __global__ __launch_bounds__(256) void k_adjoint_i8(const uint8_t *__restrict__ 
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
         const uint64_t v = wv0 + 16 * mt + li;
-        voff[mt] = (uint32_t)((16 * mt + li) * stride + 16 * lg);
-        F[mt] = (v < V && flip && flip[v]) ? 0x04040404u : 0u;
+        voff[mt] = (uint32_t)((16 * mt + li) * stride + kLaneBytes * lg);
+        const bool fl = v < V && flip && flip[v];
+        // int8: flip selector bit; BF16: all-ones mask of the code swap
+        F[mt] = fl ? (BF ? 0xFFFFFFFFu : 0x04040404u) : 0u;
     }
     const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
         (void *)frag, 0, 0x7FFFFFFF, kI8RsrcWord3);
-    auto load_ss = [&](uint64_t ss, u32x4_t (&x)[MT], u32x4_t (&b)[kBPer]) __attribute__((always_inline)) {
-        const uint32_t so = (uint32_t)(64 * ss);
+    auto load_ss = [&](uint64_t ss, RowV (&x)[MT], u32x4_t (&b)[kBPer]) __attribute__((always_inline)) {
+        const uint32_t so = (uint32_t)(kSSBytes * ss);
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-            x[mt] = __builtin_bit_cast(u32x4_t, __builtin_amdgcn_raw_buffer_load_b128(rr, voff[mt], so, 0));
+        for (int mt = 0; mt < MT; ++mt) {
+            if constexpr (BF)
+                x[mt] = __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(rr, voff[mt], so, 0));
+            else
+                x[mt] = __builtin_bit_cast(u32x4_t, __builtin_amdgcn_raw_buffer_load_b128(rr, voff[mt], so, 0));
+        }
         const uint64_t e0 = ss * (4 * TILES * 64);   // first B entry of the super-step
 #pragma unroll
         for (int m = 0; m < kBPer; ++m)
             b[m] = __builtin_bit_cast(u32x4_t, __builtin_amdgcn_raw_buffer_load_b128(
                                                    rb, (uint32_t)((e0 + tid + 256 * m) * 16), 0, 0));
     };
-    i32x4_t acc[MT][TILES];
+    AccV acc[MT][TILES];
     auto zero = [&]() __attribute__((always_inline)) {
         int z = 0;
         asm volatile("" : "+v"(z));
+        const auto e = [&] {
+            if constexpr (BF) return __builtin_bit_cast(float, z); else return z;
+        }();
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-            for (int j = 0; j < TILES; ++j) acc[mt][j] = i32x4_t{z, z, z, z};
+            for (int j = 0; j < TILES; ++j) acc[mt][j] = AccV{e, e, e, e};
     };
     zero();
     // block flush: acc[mt][j][q] is row 4g + q of tile mt (variant
@@ -330,7 +403,10 @@ __global__ __launch_bounds__(256) void k_adjoint_i8(const uint8_t *__restrict__ 
                     const uint64_t v = v0 + 16 * mt + q;
                     double sum = 0.0;
 #pragma unroll
-                    for (int j = 0; j < TILES; ++j) sum += (double)acc[mt][j][q] * sc[j];
+                    for (int j = 0; j < TILES; ++j) {
+                        const double a = acc[mt][j][q];
+                        sum += a * sc[j];
+                    }
                     if (v < V) {
                         double *d = ws + v * kp + (uint64_t)jj * 16 + lane_i;
                         *d = first ? sum : *d + sum;
@@ -342,12 +418,13 @@ __global__ __launch_bounds__(256) void k_adjoint_i8(const uint8_t *__restrict__ 
         zero();
     };
 
-    u32x4_t xa[MT], ba[kBPer];
+    RowV xa[MT];
+    u32x4_t ba[kBPer];
     load_ss(0, xa, ba);
     // outer loop over person blocks, inner loop over the block's
     // super-steps: the flush sits after the inner loop on every path, so the
     // accumulators stay in accumulation registers
-    constexpr uint64_t kSSPerBlk = kI8FlushRows / 256;
+    constexpr uint64_t kSSPerBlk = BF ? (uint64_t)kBfFlushRows / 128 : (uint64_t)kI8FlushRows / 256;
     uint64_t ss = 0;
     bool first = true;
     do {   // nss > 0
@@ -359,7 +436,7 @@ __global__ __launch_bounds__(256) void k_adjoint_i8(const uint8_t *__restrict__ 
             // be read by a wave finishing the previous super-step)
 #pragma unroll
             for (int m = 0; m < kBPer; ++m) bt[buf][tid + 256 * m] = ba[m];
-            u32x4_t xc[MT];
+            RowV xc[MT];
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) xc[mt] = xa[mt];
             lds_barrier_i8();
@@ -373,13 +450,34 @@ __global__ __launch_bounds__(256) void k_adjoint_i8(const uint8_t *__restrict__ 
                 for (int j = 0; j < TILES; ++j) bv[j] = bt[buf][(d * TILES + j) * 64 + lane];
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt) {
-                    i32x4_t A;
+                    if constexpr (BF) {
+                        // row bytes 2d, 2d+1 of the lane's 8: codes of A1
+                        // rows swapped (00 <-> 11), the two bytes duplicated,
+                        // then one v_perm_b32 per pair of BF16 dosages
+                        // (slot 2t + h: person t of byte 2d + h)
+                        uint32_t y = (uint32_t)(xc[mt] >> (32 * (d >> 1)));
+                        const uint32_t u = ~(y ^ (y >> 1)) & (F[mt] & 0x55555555u);
+                        y ^= u | (u << 1);
+                        const uint32_t P = __builtin_amdgcn_perm(y, y, (d & 1) ? 0x03030202u : 0x01010000u);
+                        BfFrag A;
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) A[t] = (int)dose_i8(((xc[mt][d] >> (2 * t)) & 0x03030303u) | F[mt]);
+                        for (int t = 0; t < 4; ++t) A.u[t] = bf_dose_pair(bf_sel_pair(P >> (2 * t)));
 #pragma unroll
-                    for (int j = 0; j < TILES; ++j) {
-                        const i32x4_t B = i32x4_t{(int)bv[j][0], (int)bv[j][1], (int)bv[j][2], (int)bv[j][3]};
-                        acc[mt][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, B, acc[mt][j], 0, 0, 0);
+                        for (int j = 0; j < TILES; ++j) {
+                            BfFrag B;
+                            B.u[0] = bv[j][0]; B.u[1] = bv[j][1]; B.u[2] = bv[j][2]; B.u[3] = bv[j][3];
+                            acc[mt][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.v, B.v, acc[mt][j], 0, 0, 0);
+                        }
+                    } else {
+                        i32x4_t A;
+#pragma unroll
+                        for (int t = 0; t < 4; ++t)
+                            A[t] = (int)dose_i8(((xc[mt][d] >> (2 * t)) & 0x03030303u) | F[mt]);
+#pragma unroll
+                        for (int j = 0; j < TILES; ++j) {
+                            const i32x4_t B = i32x4_t{(int)bv[j][0], (int)bv[j][1], (int)bv[j][2], (int)bv[j][3]};
+                            acc[mt][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, B, acc[mt][j], 0, 0, 0);
+                        }
                     }
                 }
             }
@@ -397,20 +495,25 @@ __global__ void k_adjoint_out(const double *__restrict__ ws, uint64_t V, uint64_
     if (i < V * K) dW[i] = (float)ws[(i / K) * Kp + i % K];
 }
 
-hipError_t launch_adjoint_i8(const uint8_t *rows, uint64_t stride, uint64_t V, uint64_t N, const uint8_t *flip,
-                             const uint32_t *frag, const double *scales, uint32_t tiles, uint64_t K, uint64_t Kp,
-                             uint32_t L, double *ws, float *dW, hipStream_t stream) {
+hipError_t launch_adjoint_mx(bool bf, const uint8_t *rows, uint64_t stride, uint64_t V, uint64_t N,
+                             const uint8_t *flip, const uint32_t *frag, const double *scales, uint32_t tiles,
+                             uint64_t K, uint64_t Kp, uint32_t L, double *ws, float *dW, hipStream_t stream) {
     if (V == 0) return hipSuccess;
     if ((reinterpret_cast<uintptr_t>(rows) | stride) & 3u) return hipErrorInvalidValue;
     if (stride >= (1ull << 25)) return hipErrorInvalidValue;
-#define GNM_ADJ(T, M)                                                                                              \
-    hipLaunchKernelGGL((k_adjoint_i8<T, M>), dim3((unsigned)((V + 64 * M - 1) / (64 * M))), dim3(256), 0, stream, \
-                       rows, stride, V, N, flip, (const u32x4_t *)frag, scales, Kp, L, ws)
-    if (tiles == 4) GNM_ADJ(4, 4);
-    else if (tiles == 8) GNM_ADJ(8, 2);
-    else if (tiles == 12) GNM_ADJ(12, 1);
-    else if (tiles == 16) GNM_ADJ(16, 1);
+#define GNM_ADJ(T, M, B)                                                                                          \
+    hipLaunchKernelGGL((k_adjoint_mx<T, M, B>), dim3((unsigned)((V + 64 * M - 1) / (64 * M))), dim3(256), 0,      \
+                       stream, rows, stride, V, N, flip, (const u32x4_t *)frag, scales, Kp, L, ws)
+#define GNM_ADJ_BF(T, M)                                                                                          \
+    do {                                                                                                          \
+        if (bf) GNM_ADJ(T, M, true); else GNM_ADJ(T, M, false);                                                    \
+    } while (0)
+    if (tiles == 4) GNM_ADJ_BF(4, 4);
+    else if (tiles == 8) GNM_ADJ_BF(8, 2);
+    else if (tiles == 12) GNM_ADJ_BF(12, 1);
+    else if (tiles == 16) GNM_ADJ_BF(16, 1);
     else return hipErrorInvalidValue;
+#undef GNM_ADJ_BF
 #undef GNM_ADJ
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -152,15 +152,19 @@ uint32_t bf16_splits(uint64_t num_people, uint32_t ntiles);
 hipError_t launch_forward_frag_i8(const float *W, uint64_t V, uint64_t K, uint32_t L, const int *exps,
                                   uint32_t wtiles, uint32_t tiles, uint64_t nks, uint32_t *frag, double *scales,
                                   hipStream_t stream);
-// split-int8 adjoint (gnm_i8.hip): R fragments in super-step people order,
-// then dW = G^T R through an FP64 workspace [V][Kp]; tiles = L*Kp/16 padded
-// to 4, 8, 12 or 16; rows 4-byte aligned, stride a multiple of 4 below 2^25
+// split-int8 / split-BF16 adjoint (gnm_i8.hip): R fragments in super-step
+// people order, then dW = G^T R through an FP64 workspace [V][Kp]; tiles =
+// L*Kp/16 padded to 4, 8, 12 or 16; rows 4-byte aligned, stride a multiple
+// of 4 below 2^25
 hipError_t launch_adjoint_frag_i8(const float *R, uint64_t N, uint64_t K, uint64_t Kp, uint32_t L, const int *exps,
                                   uint32_t tiles, uint32_t tiles_used, uint64_t nks, uint32_t *frag, double *scales,
                                   hipStream_t stream);
-hipError_t launch_adjoint_i8(const uint8_t *rows, uint64_t stride, uint64_t V, uint64_t N, const uint8_t *flip,
-                             const uint32_t *frag, const double *scales, uint32_t tiles, uint64_t K, uint64_t Kp,
-                             uint32_t L, double *ws, float *dW, hipStream_t stream);
+hipError_t launch_adjoint_frag_bf(const float *R, uint64_t N, uint64_t K, uint64_t Kp, uint32_t L, const int *exps,
+                                  uint32_t tiles, uint32_t tiles_used, uint64_t nks, uint32_t *frag, double *scales,
+                                  hipStream_t stream);
+hipError_t launch_adjoint_mx(bool bf, const uint8_t *rows, uint64_t stride, uint64_t V, uint64_t N,
+                             const uint8_t *flip, const uint32_t *frag, const double *scales, uint32_t tiles,
+                             uint64_t K, uint64_t Kp, uint32_t L, double *ws, float *dW, hipStream_t stream);
 // per-row selector bytes of the scoring kernel: kFlipSelI8 (int8 path) or
 // kFlipSelBf (BF16 path) for flipped rows
 constexpr uint8_t kFlipSelI8 = 4, kFlipSelBf = 0x55;

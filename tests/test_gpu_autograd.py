@@ -25,6 +25,7 @@ pytestmark = pytest.mark.gpu
 
 BLOCK = 8192   # rows (forward: variants; adjoint: people) per scale block
 I8_BLOCK = 65536   # variants per scale block of the split-int8 forward
+BF_ADJ_BLOCK = 32768   # people per scale block of the split-limb BF16 adjoint
 LIMBS = {"bf16x4": 4, "bf16x3": 3, "bf16x2": 2, "i8x4": 4, "i8x3": 3}
 
 
@@ -95,29 +96,35 @@ def test_dense_forward_matches_fp64(orc, precision):
             assert err.max() / np.abs(S64).max() <= 1e-6
 
 
-@pytest.mark.parametrize("precision,K,N,V", [("bf16x3", 16, 9000, 4100), ("bf16x3", 40, 9000, 4100),
-                                              ("bf16x2", 16, 9000, 4100), ("i8x4", 16, 9000, 4100),
-                                              ("i8x4", 40, 9000, 4100), ("i8x3", 16, 9000, 4100),
-                                              ("i8x4", 64, 9000, 300), ("i8x4", 16, 70_000, 300)])
-def test_adjoint_matches_fp64(orc, precision, K, N, V):
-    """N = 9000: two BF16 person blocks, one int8 block, a last super-step
-    that runs past the row end, partial last workgroup; N = 70,000: two int8
-    person blocks (FP64 flush across blocks); K = 64: 16 int8 tiles."""
-    _flush(f"adjoint {precision} N={N} V={V} K={K}")
+@pytest.mark.parametrize("precision,K,N,V,pad", [
+    ("bf16x3", 16, 9000, 4100, False), ("bf16x3", 40, 9000, 4100, False), ("bf16x2", 16, 9000, 4100, False),
+    ("bf16x4", 16, 9000, 4100, True), ("bf16x3", 40, 9000, 4100, True), ("bf16x4", 64, 9000, 300, True),
+    ("bf16x4", 16, 70_000, 300, True), ("bf16x2", 8, 9000, 300, True),
+    ("i8x4", 16, 9000, 4100, True), ("i8x4", 40, 9000, 4100, True), ("i8x3", 16, 9000, 4100, True),
+    ("i8x4", 64, 9000, 300, True), ("i8x4", 16, 70_000, 300, True)])
+def test_adjoint_matches_fp64(orc, precision, K, N, V, pad):
+    """Rows padded to a multiple of 4 bytes run the split-limb adjoint kernel
+    (int8 and BF16); 2250-byte rows run the byte-load BF16 kernel.  N = 9000:
+    a last super-step that runs past the row end, a partial last workgroup,
+    two person blocks of the byte-load kernel; N = 70,000: two person blocks
+    of the split-limb kernel (an FP64 flush across blocks); K = 64: 16
+    tiles."""
+    _flush(f"adjoint {precision} N={N} V={V} K={K} pad={pad}")
     dos, rows, flip, _ = _data(N, V, 1, 12)
     rng = np.random.default_rng(13)
     R = rng.normal(0, 1, (N, K)).astype(np.float32)
     R[:, 3] *= 1e-6                    # a small column: scales are per column
     dW64 = orc.adjoint_fp64(rows, N, flip, R.astype(np.float64))
-    if precision.startswith("i8"):
-        # the int8 adjoint reads 4-byte aligned rows: pad each row with zero bytes
+    if pad:
+        # the split-limb adjoint reads 4-byte aligned rows: pad each row with zero bytes
         rows = np.pad(rows, ((0, 0), (0, -rows.shape[1] % 4)))
         assert ag._adjoint_precision(precision, _dev(rows)) == precision
     dW = ag.dense_adjoint(_dev(R), _dev(rows), _dev(flip), V, precision)
     torch.cuda.synchronize()
     dW = dW.cpu().numpy().astype(np.float64)
     G = _effect_dosage(dos, flip).T                   # [N, V]
-    bound = _limb_bound(G, R.astype(np.float64), precision, I8_BLOCK if precision.startswith("i8") else BLOCK)
+    block = I8_BLOCK if precision.startswith("i8") else (BF_ADJ_BLOCK if pad else BLOCK)
+    bound = _limb_bound(G, R.astype(np.float64), precision, block)
     err = np.abs(dW - dW64)
     assert np.all(err <= bound + 2.0 ** -23 * np.abs(dW64) + 1e-30), (err - bound).max()
 
@@ -128,7 +135,9 @@ def test_int8_adjoint_falls_back_on_unaligned_rows():
 
 
 def test_adjoint_rejects_too_many_scores():
-    N, V, K = 256, 64, 65
+    # 100 scores in bf16x3 are 21 limb tiles: above the 16 of the split-limb
+    # kernel and the 12 of the byte-load kernel
+    N, V, K = 256, 64, 100
     rows = np.zeros((V, N // 4), np.uint8)
     R = torch.zeros(N, K, dtype=torch.float32, device="cuda")
     with pytest.raises(_lib.GnmError):
