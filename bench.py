@@ -67,18 +67,13 @@ def cpu_facts():
     return facts
 
 
-def mfma_work(precision, N, V, K, nclass=1):
+def mfma_work(precision, N, program_rows, tiles):
     """Executed matrix operations of one scoring pass of a split path: the
     kernel multiplies every (person, program row) dosage by every column of
-    its 16-column tiles, 2 operations (multiply, add) each.  Program rows = V
-    here (flips are per variant, so no variant needs two rows); tiles =
-    ceil(limbs * K / 16) weight tiles, plus ceil(nclass / 16) count tiles
-    when scores do not share one membership class (one class is counted on
-    the vector ALUs)."""
-    limbs = int(precision[-1])
-    tiles = (limbs * K + 15) // 16 + (0 if nclass == 1 else (nclass + 15) // 16)
+    its 16-column tiles (weight limbs plus missing counts, as the library
+    reports them), 2 operations (multiply, add) each."""
     kind = "int8" if precision.startswith("i8") else "bf16"
-    return 2 * N * V * tiles * 16, kind, tiles
+    return 2 * N * program_rows * tiles * 16, kind
 
 
 def log(msg):
@@ -671,6 +666,7 @@ def main():
         return ctx, ms_step, float(np.mean(main_ms))
 
     ctx, ms_step, main_ms = run_path(args.precision)
+    prog_rows, mfma_tiles = ctx.launch_shape()
     svs = N * V * K
     value = svs / (ms_step / 1e3)
 
@@ -710,7 +706,8 @@ def main():
         roofline = {"bound": "hbm", "achieved": round(hbm_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(hbm_gbs / HBM_PEAK_GBS, 4)}
     else:
-        ops, kind, tiles = mfma_work(args.precision, N, V, K)
+        ops, kind = mfma_work(args.precision, N, prog_rows, mfma_tiles)
+        tiles = mfma_tiles
         tflops = ops / (main_ms / 1e3) / 1e12
         peak = MFMA_PEAK_TFLOPS[kind]
         roofline = {"bound": "mfma", "achieved": round(tflops, 1), "peak": peak,

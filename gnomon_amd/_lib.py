@@ -35,7 +35,7 @@ EXPORTS = (
     "gnm_create_forced", "gnm_score_rows", "gnm_score_rows_device", "gnm_read_accumulators",
     "gnm_seed_accumulators", "gnm_finish", "gnm_destroy", "gnm_run", "gnm_unpack_device",
     "gnm_dense_forward_device", "gnm_dense_adjoint_device", "gnm_dense_adjoint_device_noalloc",
-    "gnm_synth_rows_device", "gnm_hbm_probe",
+    "gnm_synth_rows_device", "gnm_hbm_probe", "gnm_context_shape",
     "gnm_last_timing", "gnm_set_launch_config", "gnm_set_sparse_ingest", "gnm_ingest_stats",
     "gnm_sparse_encode_row",
 )
@@ -116,6 +116,8 @@ def lib():
         f.argtypes = [vp, u64, u64, u64, vp, vp, u64, i32, vp, vp]
     L.gnm_synth_rows_device.restype = st
     L.gnm_synth_rows_device.argtypes = [u64, u32, u32, u32, u64, u64, u64, vp, u64, vp]
+    L.gnm_context_shape.restype = st
+    L.gnm_context_shape.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u32)]
     L.gnm_hbm_probe.restype = st
     L.gnm_hbm_probe.argtypes = [i32, vp, u64, vp, u64, vp]
     L.gnm_set_launch_config.restype = st

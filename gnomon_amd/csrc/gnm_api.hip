@@ -1159,6 +1159,13 @@ gnm_status gnm_set_launch_config(gnm_ctx *ctx, uint32_t tiles_per_pass, uint32_t
     return GNM_OK;
 }
 
+gnm_status gnm_context_shape(const gnm_ctx *ctx, uint64_t *program_rows, uint32_t *mfma_tiles) {
+    if (!ctx || !program_rows || !mfma_tiles) return fail(GNM_ERR_INVALID_ARG, "NULL argument");
+    *program_rows = ctx->R;
+    *mfma_tiles = ctx->precision == GNM_PRECISION_FP32 ? 0u : ctx->plan.wtiles + ctx->plan.ctiles;
+    return GNM_OK;
+}
+
 gnm_status gnm_set_sparse_ingest(gnm_ctx *ctx, double max_word_fraction) {
     if (!ctx) return fail(GNM_ERR_INVALID_ARG, "ctx is NULL");
     if (!(max_word_fraction >= 0.0 && max_word_fraction <= 0.5))

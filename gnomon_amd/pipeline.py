@@ -67,6 +67,13 @@ class ScoringContext:
         if st != _lib.GNM_OK:
             raise GnmError(st, _lib.last_error())
 
+    def launch_shape(self):
+        """(program rows, MFMA tiles per row) of the scoring launch; one pass
+        executes 2 * N * rows * 16 * tiles matrix operations (tiles 0: FP32 path)."""
+        rows, tiles = ctypes.c_uint64(), ctypes.c_uint32()
+        check(lib().gnm_context_shape(self._ctx, ctypes.byref(rows), ctypes.byref(tiles)))
+        return int(rows.value), int(tiles.value)
+
     # -- rows ---------------------------------------------------------------
     def score_rows(self, rows: np.ndarray, first_variant: int) -> None:
         rows = np.ascontiguousarray(rows, dtype=np.uint8)

@@ -207,6 +207,14 @@ gnm_status gnm_unpack_device(const uint8_t *d_rows, uint64_t bytes_per_variant,
                              uint64_t num_variants, uint64_t num_people, const uint8_t *d_flip,
                              uint8_t *d_dosage, uint32_t *d_missing_per_person, void *stream);
 
+/* Shape of the context's scoring launch, for roofline accounting (no
+ * reference counterpart): program rows (variants with CSR entries; a variant
+ * whose cells mix flipped and unflipped effect alleles counts twice) and the
+ * 16-column MFMA tiles each row is multiplied by (weight limbs plus missing
+ * counts; 0 on the FP32 path).  One pass over all rows executes
+ * 2 * num_people * program_rows * 16 * mfma_tiles matrix operations. */
+gnm_status gnm_context_shape(const gnm_ctx *ctx, uint64_t *program_rows, uint32_t *mfma_tiles);
+
 /* HBM ceiling probes for the unpack stage's roofline (no reference
  * counterpart: measurement only).  kind 0 reads src_bytes from d_src (output:
  * one dword per probe thread, d_dst of at least 4 MiB); kind 1 writes
