@@ -559,11 +559,17 @@ def cpu_baseline(d_bed, csr, N, V, K, bpv, stride, target_s=15.0):
     t1 = time.perf_counter()
     orc.refcpu_run(rows, head(vs), N, threads=threads)
     rate = N * vs * K / (time.perf_counter() - t1)
-    vs = int(min(V, max(2000, rate * target_s / (N * K))))
-    rows = d_bed.view(V, stride)[:vs, :bpv].cpu().numpy()
-    t1 = time.perf_counter()
-    _, _, used = orc.refcpu_run(rows, head(vs), N, threads=threads)
-    dt = time.perf_counter() - t1
+    # the first, short pass carries start-up costs and under-estimates the
+    # rate: a sample that ran under half the target is resized and run again
+    for _ in range(3):
+        vs = int(min(V, max(2000, rate * target_s / (N * K))))
+        rows = d_bed.view(V, stride)[:vs, :bpv].cpu().numpy()
+        t1 = time.perf_counter()
+        _, _, used = orc.refcpu_run(rows, head(vs), N, threads=threads)
+        dt = time.perf_counter() - t1
+        rate = N * vs * K / dt
+        if dt >= 0.5 * target_s or vs == V:
+            break
     out = {"value": N * vs * K / dt, "unit": "sample*variant*score/s", "cores": used, "kind": "port",
            "sample": f"restated reference CPU engine (oracle/gnomon_oracle.c) on {N} people x {vs} variants x "
                      f"{K} scores of the same workload, {dt:.1f} s",
