@@ -1125,7 +1125,7 @@ gnm_status gnm_hbm_probe(int kind, const void *d_src, uint64_t src_bytes, void *
         if (src_bytes == 0 || src_bytes % 16) return fail(GNM_ERR_INVALID_ARG, "probe size must be a positive multiple of 16");
         if (!d_dst || (b & 15u)) return fail(GNM_ERR_INVALID_ARG, "probe output must be a 16-byte aligned device pointer");
         if (kind != 1 && (!d_src || (a & 15u))) return fail(GNM_ERR_INVALID_ARG, "probe input must be a 16-byte aligned device pointer");
-        const uint64_t need = kind == 0 ? (uint64_t)256 * 16 * 256 * 4 : (kind == 1 ? src_bytes : 4 * src_bytes);
+        const uint64_t need = kind == 0 ? hbm_probe_sink_bytes() : (kind == 1 ? src_bytes : 4 * src_bytes);
         if (dst_bytes < need) return fail(GNM_ERR_INVALID_ARG, "probe output buffer too small");
         GNM_HIP(launch_hbm_probe(kind, (const uint8_t *)d_src, src_bytes, (uint8_t *)d_dst, (hipStream_t)stream));
         return GNM_OK;

@@ -98,6 +98,7 @@ hipError_t launch_synth(uint64_t seed, uint32_t p_lo24, uint32_t p_hi24, uint32_
                         uint64_t num_samples, uint64_t first_variant, uint64_t num_rows,
                         uint8_t *out, uint64_t row_stride, hipStream_t stream);
 hipError_t launch_hbm_probe(int kind, const uint8_t *src, uint64_t src_bytes, uint8_t *dst, hipStream_t stream);
+uint64_t hbm_probe_sink_bytes();   // output bytes the read probe writes (one dword per thread)
 hipError_t launch_unpack(const uint8_t *rows, uint64_t stride, uint64_t num_rows,
                          uint64_t num_people, const uint8_t *flip, uint8_t *dosage,
                          uint32_t *missing, hipStream_t stream);

@@ -100,12 +100,17 @@ hipError_t launch_synth(uint64_t seed, uint32_t p_lo24, uint32_t p_hi24, uint32_
 // range check.  Stores are non-temporal: the tiles are not re-read by this
 // kernel.
 // ---------------------------------------------------------------------------
-constexpr int kUnpackBatch = 8;          // rows loaded before any decode
+// rows loaded before any decode (a multiple of 8, dividing kUnpackRows):
+// 100k people x 1M rows took 27.1 ms with 8, 23.5 with 16, 22.2 with 24
+// and 21.6 with 32 -- more loads in flight per wave keep HBM busier
+constexpr int kUnpackBatch = 32;
 constexpr int kUnpackBlock = 256;        // 4 waves
 constexpr int kUnpackRows = 512;         // rows per workgroup (<= 65535 for the pair counters)
 constexpr int kUnpackBytesRows = 2048;   // rows per workgroup of the byte-wise path
 constexpr int kRsrcWord3 = 0x00020000;   // gfx9 raw buffer descriptor: 32-bit data format
 constexpr int kStoreNT = 2;              // buffer store cache policy: non-temporal
+static_assert(kUnpackBatch % 8 == 0 && kUnpackRows % kUnpackBatch == 0 && kUnpackBatch < 256,
+              "whole flag words per batch, whole batches per workgroup, byte counters drained in time");
 
 __device__ __forceinline__ uint32_t spread_fields(uint32_t byte4) {
     // four 2-bit fields of the low byte -> four bytes
@@ -193,11 +198,13 @@ __global__ __launch_bounds__(kUnpackBlock) void k_unpack(const uint8_t *__restri
         uint32_t w[R][4];
 #pragma unroll
         for (int i = 0; i < R; ++i) load(w[i], r + i);
-        // flip flags of the batch: one uniform 8-byte load (r % 8 == 0)
-        const uint64_t fw = flip ? *(const uint64_t *)(flip + r) : 0;
+        // flip flags of the batch: uniform 8-byte loads (r % 8 == 0)
+        uint64_t fw[R / 8];
+#pragma unroll
+        for (int q = 0; q < R / 8; ++q) fw[q] = flip ? *(const uint64_t *)(flip + r + 8 * q) : 0;
         __builtin_amdgcn_sched_barrier(0);   // all loads of the batch issue before its first store
 #pragma unroll
-        for (int i = 0; i < R; ++i) row(w[i], 0u - (uint32_t)(((fw >> (8 * i)) & 0xFFu) != 0), r + i);
+        for (int i = 0; i < R; ++i) row(w[i], 0u - (uint32_t)(((fw[i / 8] >> (8 * (i % 8))) & 0xFFu) != 0), r + i);
         since += R;
         if (since + R > 255) {
             drain();
@@ -727,25 +734,31 @@ hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t value, hipStream_t 
 // HBM ceiling probes: the access shapes of the unpack stage without its
 // decode, so that its rate can be read against what the memory system
 // delivers for the same shapes.  Grid-stride over 16-byte elements (n16 =
-// bytes / 16 of the source) or dwords (n4).
-//   read:   16-byte buffer loads, folded into one dword per thread
+// bytes / 16 of the source) or dwords (n4), kProbeInFlight independent
+// accesses per thread and iteration.
+//   read:   16-byte non-temporal loads, folded into one dword per thread
 //   write:  16-byte non-temporal buffer stores (the unpack stage's stores)
 //   expand: every source dword becomes 16 bytes of output (1:4, the
 //           unpack stage's ratio of packed bytes to tile bytes)
-// Each thread keeps four independent accesses in flight per iteration.
+// Shape: a sweep over 16 GiB (grids of 1024 to 65536 workgroups of 256
+// threads, one or four stores per iteration, plain or non-temporal) put
+// writes at 4.4-5.8 TB/s and reads at 6.5-7.0 TB/s; 16384 workgroups with
+// eight reads or four writes in flight gave the top of both ranges.
+constexpr int kProbeGrid = 16384;
+constexpr int kProbeInFlight = 8;
 __global__ __launch_bounds__(256) void k_probe_read(const uint8_t *__restrict__ src, uint64_t n16,
                                                      uint32_t *__restrict__ sink) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (uint64_t)gridDim.x * blockDim.x;
     u32x4_t acc = {0u, 0u, 0u, 0u};
-    for (uint64_t i = t; i < n16; i += 4 * nt) {
-        u32x4_t v[4];
+    for (uint64_t i = t; i < n16; i += kProbeInFlight * nt) {
+        u32x4_t v[kProbeInFlight];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kProbeInFlight; ++u) {
             const uint64_t j = i + u * nt;
-            v[u] = j < n16 ? *(const u32x4_t *)(src + 16 * j) : u32x4_t{0u, 0u, 0u, 0u};
+            v[u] = j < n16 ? __builtin_nontemporal_load((const u32x4_t *)(src + 16 * j)) : u32x4_t{0u, 0u, 0u, 0u};
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc ^= v[u];
+        for (int u = 0; u < kProbeInFlight; ++u) acc ^= v[u];
     }
     sink[t] = acc[0] ^ acc[1] ^ acc[2] ^ acc[3];
 }
@@ -766,15 +779,15 @@ __global__ __launch_bounds__(256) void k_probe_write(uint8_t *__restrict__ dst, 
 __global__ __launch_bounds__(256) void k_probe_expand(const uint8_t *__restrict__ src, uint64_t n4,
                                                        uint8_t *__restrict__ dst) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = t; i < n4; i += 4 * nt) {
-        uint32_t x[4];
+    for (uint64_t i = t; i < n4; i += kProbeInFlight * nt) {
+        uint32_t x[kProbeInFlight];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kProbeInFlight; ++u) {
             const uint64_t j = i + u * nt;
-            x[u] = j < n4 ? *(const uint32_t *)(src + 4 * j) : 0u;
+            x[u] = j < n4 ? __builtin_nontemporal_load((const uint32_t *)(src + 4 * j)) : 0u;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kProbeInFlight; ++u) {
             const uint64_t j = i + u * nt;
             if (j >= n4) break;
             const u32x4_t o = {x[u] & 0x03030303u, (x[u] >> 2) & 0x03030303u, (x[u] >> 4) & 0x03030303u,
@@ -786,8 +799,10 @@ __global__ __launch_bounds__(256) void k_probe_expand(const uint8_t *__restrict_
     }
 }
 
+uint64_t hbm_probe_sink_bytes() { return (uint64_t)kProbeGrid * 256 * 4; }
+
 hipError_t launch_hbm_probe(int kind, const uint8_t *src, uint64_t src_bytes, uint8_t *dst, hipStream_t stream) {
-    const dim3 grid(256 * 16), block(256);   // 16 workgroups per CU, grid-stride
+    const dim3 grid(kProbeGrid), block(256);   // 64 workgroups per CU, grid-stride
     if (kind == 0)
         hipLaunchKernelGGL(k_probe_read, grid, block, 0, stream, src, src_bytes / 16, (uint32_t *)dst);
     else if (kind == 1)

@@ -217,7 +217,7 @@ gnm_status gnm_context_shape(const gnm_ctx *ctx, uint64_t *program_rows, uint32_
 
 /* HBM ceiling probes for the unpack stage's roofline (no reference
  * counterpart: measurement only).  kind 0 reads src_bytes from d_src (output:
- * one dword per probe thread, d_dst of at least 4 MiB); kind 1 writes
+ * one dword per probe thread, d_dst of at least 16 MiB); kind 1 writes
  * src_bytes to d_dst with the unpack stage's non-temporal 16-byte stores
  * (d_src unused); kind 2 expands every source dword into 16 output bytes
  * (reads src_bytes, writes 4 * src_bytes), the unpack stage's 1:4 ratio.

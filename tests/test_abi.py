@@ -96,7 +96,7 @@ def test_hbm_probe_rejects_bad_arguments_without_device():
     assert L.gnm_hbm_probe(2, buf, 4096, ctypes.c_void_p((1 << 20) + 4), 1 << 30, None) == _lib.GNM_ERR_INVALID_ARG
     assert L.gnm_hbm_probe(2, buf, 4096, buf, 4 * 4096 - 16, None) == _lib.GNM_ERR_INVALID_ARG
     assert "too small" in _lib.last_error()
-    assert L.gnm_hbm_probe(0, buf, 4096, buf, 1 << 20, None) == _lib.GNM_ERR_INVALID_ARG   # sink < 4 MiB
+    assert L.gnm_hbm_probe(0, buf, 4096, buf, (16 << 20) - 16, None) == _lib.GNM_ERR_INVALID_ARG   # sink < 16 MiB
     assert L.gnm_hbm_probe(0, None, 4096, buf, 1 << 30, None) == _lib.GNM_ERR_INVALID_ARG
 
 

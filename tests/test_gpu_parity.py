@@ -97,7 +97,7 @@ def test_synth_device_matches_cpu_generator(orc):
 def test_unpack_bit_exact(orc, N, V, with_flip):
     """Vector path (N a multiple of 16; 4112 people = one full wave span plus
     16 people), byte path (N odd), several row blocks (V > 512), row counts
-    that are not a multiple of the 8-row batch, no flip array."""
+    that are not a multiple of the 32-row batch, no flip array."""
     _flush(f"unpack N={N} V={V} flip={with_flip}")
     rng = np.random.default_rng(N)
     dos = rng.choice([0, 1, 2, -1], size=(V, N), p=[0.45, 0.3, 0.2, 0.05])
