@@ -165,6 +165,9 @@ def measure_unpack(d_bed, N, V, bpv, stride, reps, workload):
     return {"kernel": "k_unpack", "variants": vu, "people": N, "ms": round(ms, 4), "algorithmic_bytes": alg,
             "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
             "measured_ceilings_gbs": ceil,
+            # the best rate any probe moved on this device (the measured HBM
+            # peak the unpack target is stated against) and the 1:4 copy's
+            "frac_of_measured_peak": round(gbs / max(ceil.values()), 4) if ceil else None,
             "frac_of_expand_ceiling": round(gbs / ceil["expand_1to4"], 4) if ceil else None,
             "traffic": traffic, "traffic_source": src}
 
