@@ -562,11 +562,14 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
             P[2 * q] = __builtin_amdgcn_perm(y, y, 0x01010000u);
             P[2 * q + 1] = __builtin_amdgcn_perm(y, y, 0x03030202u);
         }
+        // B fragments of the first k-step; those of the second are read
+        // into the same registers right after each one's last MFMA of the
+        // first k-step, so their LDS latency runs under the remaining MFMAs
+        u32x4_t bv[TMAX];
+#pragma unroll
+        for (int j = 0; j < TMAX; ++j) bv[j] = btile(par)[buf][((set * TMAX + j) * 2) * 64 + lane];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            u32x4_t bv[TMAX];
-#pragma unroll
-            for (int j = 0; j < TMAX; ++j) bv[j] = btile(par)[buf][((set * TMAX + j) * 2 + h) * 64 + lane];
 #pragma unroll
             for (int tt = 0; tt < 4; ++tt) {
                 BfFrag A, M;
@@ -589,7 +592,10 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
                     }
                     __builtin_amdgcn_s_setprio(1);
 #pragma unroll
-                    for (int j = 0; j < TMAX; ++j) mfma(j, j < NW ? A : M);
+                    for (int j = 0; j < TMAX; ++j) {
+                        mfma(j, j < NW ? A : M);
+                        if (h == 0 && tt == 3) bv[j] = btile(par)[buf][((set * TMAX + j) * 2 + 1) * 64 + lane];
+                    }
                     __builtin_amdgcn_s_setprio(0);
                 } else {
 #pragma unroll
@@ -603,6 +609,10 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX
 #pragma unroll
                         for (int j = 0; j < TMAX; ++j)
                             if ((uint32_t)j >= nw && (uint32_t)j < ntile) mfma(j, M);
+                    }
+                    if (h == 0 && tt == 3) {
+#pragma unroll
+                        for (int j = 0; j < TMAX; ++j) bv[j] = btile(par)[buf][((set * TMAX + j) * 2 + 1) * 64 + lane];
                     }
                 }
             }
