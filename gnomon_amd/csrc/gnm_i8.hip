@@ -305,8 +305,11 @@ hipError_t launch_adjoint_frag_bf(const float *R, uint64_t N, uint64_t K, uint64
 // fragments of its 4 k-steps (16 B x TILES each) and store them to LDS
 // (double-buffered, one barrier per super-step); each lane loads its MT
 // rows' bytes.  Both are loaded one super-step ahead into registers.
+// The 4-tile BF16 instance is held to 168 registers (three waves per SIMD):
+// at 50k x 500k x 16 in bf16x4 3.16 against 3.43 ms with two; the int8
+// instance ran slower that way (1.93 against 1.78 ms) and keeps two.
 template <int TILES, int MT, bool BF>
-__global__ __launch_bounds__(256) void k_adjoint_mx(const uint8_t *__restrict__ rows, uint64_t stride, uint64_t V,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TILES == 4 && BF ? 3 : 1))) void k_adjoint_mx(const uint8_t *__restrict__ rows, uint64_t stride, uint64_t V,
                                                     uint64_t N, const uint8_t *__restrict__ flip,
                                                     const u32x4_t *__restrict__ frag,
                                                     const double *__restrict__ scales, uint64_t Kp, uint32_t L,
