@@ -473,8 +473,13 @@ __device__ __forceinline__ void fp32_row(uint32_t b, uint32_t flips, uint32_t cn
 // wave loads 16 rows' variant indices into lanes and reads them back one by
 // one.  Rows past the split read its last row; they are multiplied with
 // weight 0, no flip and no count.
+// Held to 128 registers (four waves per SIMD; the FP64 sums and the
+// accumulators then share the VGPRs, a few values outside the row loop are
+// spilled): at 100k x 1M x 8 27.0 against 28.1 ms with three.  The
+// multi-class count instances keep the compiler's choice (their count
+// registers would spill inside the loop).
 template <int KC, int NCLS, bool IDENT>
-__global__ __launch_bounds__(256) void k_score_fp32(DevicePlan plan, const uint8_t *__restrict__ rows,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NCLS <= 1 || KC == 4 ? 4 : 1))) void k_score_fp32(DevicePlan plan, const uint8_t *__restrict__ rows,
                                                     uint64_t stride, uint64_t bpv, uint64_t v0, uint64_t p0,
                                                     uint64_t p1, uint64_t rps, uint32_t npass, uint32_t npb,
                                                     uint32_t nsplit, uint64_t num_people, uint64_t num_scores,
