@@ -145,14 +145,14 @@ __device__ __forceinline__ uint32_t bf_miss_pair(uint32_t x) {
 template <int TMAX, int WAVES, int D, bool VEC, bool IDENT, int SETS, bool BF>
 // The 3-tile, 2-stage instances (int8 and BF16) are asked to fit four waves
 // per SIMD (128 registers; a few values outside the MFMA blocks are spilled),
-// the 5-tile BF16 and the 4- and 5-tile int8 instances three (168
+// the 2-stage 5-tile BF16 and 4- and 5-tile int8 instances three (168
 // registers, three workgroups per CU; at 50k x 500k x 16 the 5-tile groups
 // took 3.80 against 4.28 ms in bf16x4 and 2.33 against 2.72 ms in i8x4 with
 // two), the other BF16 instances two waves per SIMD (256 registers).
 // SETS > 1: the waves form SETS sets over the same people, each set with its
 // own group of TMAX tiles, so one row read serves SETS groups (the first
 // set loads and stores the rows for all).
-__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX == 3 && D == 2 ? 4 : (TMAX == 5 || (!BF && TMAX == 4) ? 3 : (BF ? 2 : 1))))) void k_score_mx(DevicePlan plan, const uint8_t *__restrict__ rows,
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(TMAX == 3 && D == 2 ? 4 : ((TMAX == 5 || (!BF && TMAX == 4)) && D == 2 ? 3 : (BF ? 2 : 1))))) void k_score_mx(DevicePlan plan, const uint8_t *__restrict__ rows,
                                                          uint64_t stride, uint64_t bpv, uint64_t v0, uint64_t p0,
                                                          uint64_t p1, uint64_t split_base, uint64_t rps,
                                                          uint32_t splits, uint32_t person_tiles,
