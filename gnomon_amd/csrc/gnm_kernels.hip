@@ -101,8 +101,8 @@ hipError_t launch_synth(uint64_t seed, uint32_t p_lo24, uint32_t p_hi24, uint32_
 // kernel.
 // ---------------------------------------------------------------------------
 // rows loaded before any decode (a multiple of 8, dividing kUnpackRows):
-// 100k people x 1M rows took 27.1 ms with 8, 23.5 with 16, 22.2 with 24
-// and 21.6 with 32 -- more loads in flight per wave keep HBM busier
+// 100k people x 1M rows took 27.1 ms with 8 and 26.2 with 32; with the row
+// blocks varying fastest over the grid, 16, 24 and 32 all ran 24.7-25.2 ms
 constexpr int kUnpackBatch = 32;
 constexpr int kUnpackBlock = 256;        // 4 waves
 constexpr int kUnpackRows = 512;         // rows per workgroup (<= 65535 for the pair counters)
