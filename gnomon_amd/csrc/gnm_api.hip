@@ -1109,6 +1109,9 @@ gnm_status gnm_unpack_device(const uint8_t *d_rows, uint64_t bytes_per_variant, 
         if (num_variants && (!d_rows || !d_dosage || !d_missing_per_person))
             return fail(GNM_ERR_INVALID_ARG, "NULL device pointer");
         if (bytes_per_variant < (num_people + 3) / 4) return fail(GNM_ERR_INVALID_ARG, "bytes_per_variant too small");
+        // the launch grid holds at most 65535 spans of 16384 people
+        if (num_people > 65535ull * 16384ull)
+            return fail(GNM_ERR_INVALID_ARG, "num_people exceeds the unpack limit of 1073725440");
         GNM_HIP(launch_unpack(d_rows, bytes_per_variant, num_variants, num_people, d_flip, d_dosage,
                               d_missing_per_person, (hipStream_t)stream));
         return GNM_OK;

@@ -201,8 +201,9 @@ gnm_status gnm_run(const gnm_problem *problem, const uint8_t *rows, uint64_t row
 /* Standalone unpack: dosage[v*num_people + p] = reference tile value
  * (score/batch.rs:489-503): effect-allele dosage 0/1/2 (2-d when flip[v]),
  * 3 = missing; missing_per_person[p] += number of missing calls in the block
- * (accumulated, caller zeroes).  flip may be NULL.  Replaces unpack_plink
- * (cuda_backend.rs:70-125) as a separate stage. */
+ * (accumulated, caller zeroes).  flip may be NULL.  num_people at most
+ * 1073725440.  Replaces unpack_plink (cuda_backend.rs:70-125) as a separate
+ * stage. */
 gnm_status gnm_unpack_device(const uint8_t *d_rows, uint64_t bytes_per_variant,
                              uint64_t num_variants, uint64_t num_people, const uint8_t *d_flip,
                              uint8_t *d_dosage, uint32_t *d_missing_per_person, void *stream);

@@ -86,6 +86,14 @@ def test_invalid_arguments_rejected_without_device():
     assert not ctx.value
 
 
+def test_unpack_rejects_more_people_than_its_grid_holds_without_device():
+    L = _lib.lib()
+    buf = ctypes.c_void_p(1 << 20)   # never dereferenced: the call fails its checks first
+    N = 65535 * 16384 + 16
+    assert L.gnm_unpack_device(buf, (N + 3) // 4, 1, N, None, buf, buf, None) == _lib.GNM_ERR_INVALID_ARG
+    assert "limit" in _lib.last_error()
+
+
 def test_hbm_probe_rejects_bad_arguments_without_device():
     L = _lib.lib()
     buf = ctypes.c_void_p(1 << 20)   # never dereferenced: every call below fails its checks first
