@@ -692,6 +692,11 @@ uint32_t fp32_splits(uint64_t num_people, uint64_t num_scores) {
 // Cross-split FP64 reduction into the persistent accumulators (fixed order,
 // deterministic), and the per-score count gather from membership classes.
 // ---------------------------------------------------------------------------
+// One thread per (person, score); the sum runs split by split, limb by limb
+// inside a split, the same order on every launch.  LC > 0: the limb count
+// at compile time, and four splits' loads are issued before their adds (the
+// order of the adds is unchanged), so more loads are in flight per thread.
+template <int LC>
 __global__ void k_reduce(const double *__restrict__ part, const uint32_t *__restrict__ cpart,
                          uint32_t splits, uint64_t num_people, uint64_t num_scores, uint32_t limbs,
                          uint32_t nclass, const uint32_t *__restrict__ score_class,
@@ -699,13 +704,34 @@ __global__ void k_reduce(const double *__restrict__ part, const uint32_t *__rest
     const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= num_people * num_scores) return;
     const uint64_t p = idx / num_scores, k = idx % num_scores;
-    const uint64_t ncol = (uint64_t)limbs * num_scores;
+    const uint32_t L = LC > 0 ? (uint32_t)LC : limbs;
+    const uint64_t ncol = (uint64_t)L * num_scores;
     double acc = 0.0;
     uint32_t cnt = 0;
     const uint32_t cls = score_class[k];
-    for (uint32_t s = 0; s < splits; ++s) {
+    uint32_t s = 0;
+    if constexpr (LC > 0) {
+        for (; s + 4 <= splits; s += 4) {
+            double v[4][LC];
+            uint32_t c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double *ps = part + ((uint64_t)(s + u) * num_people + p) * ncol + k;
+#pragma unroll
+                for (int l = 0; l < LC; ++l) v[u][l] = ps[(uint64_t)l * num_scores];
+                c[u] = cpart[((uint64_t)(s + u) * num_people + p) * nclass + cls];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+#pragma unroll
+                for (int l = 0; l < LC; ++l) acc += v[u][l];
+                cnt += c[u];
+            }
+        }
+    }
+    for (; s < splits; ++s) {
         const double *ps = part + ((uint64_t)s * num_people + p) * ncol + k;
-        for (uint32_t l = 0; l < limbs; ++l) acc += ps[(uint64_t)l * num_scores];
+        for (uint32_t l = 0; l < L; ++l) acc += ps[(uint64_t)l * num_scores];
         cnt += cpart[((uint64_t)s * num_people + p) * nclass + cls];
     }
     scores[idx] += acc;
@@ -718,8 +744,16 @@ hipError_t launch_reduce(const double *part, const uint32_t *cpart, uint32_t spl
                          uint32_t *counts, hipStream_t stream) {
     const uint64_t n = num_people * num_scores;
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, part, cpart,
-                       splits, num_people, num_scores, limbs, nclass, score_class, scores, counts);
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+#define GNM_REDUCE(LC)                                                                                       \
+    hipLaunchKernelGGL((k_reduce<LC>), grid, block, 0, stream, part, cpart, splits, num_people, num_scores, \
+                       limbs, nclass, score_class, scores, counts)
+    if (limbs == 1) GNM_REDUCE(1);
+    else if (limbs == 2) GNM_REDUCE(2);
+    else if (limbs == 3) GNM_REDUCE(3);
+    else if (limbs == 4) GNM_REDUCE(4);
+    else GNM_REDUCE(0);
+#undef GNM_REDUCE
     return hipGetLastError();
 }
 
